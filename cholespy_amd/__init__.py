@@ -1,0 +1,22 @@
+"""cholespy_amd: cholespy's API with an MI355X-native (gfx950) GPU solve phase.
+
+Exports the same names as cholespy (MatrixType, CholeskySolverF,
+CholeskySolverD, __version__). The native module is built in-tree
+(`python cholespy_amd/build.py`); importing without it raises ImportError,
+there is no Python fallback.
+"""
+from importlib import import_module as _import_module
+
+try:
+    _core = _import_module("cholespy_amd._cholespy_core")
+except ImportError as exc:  # pragma: no cover - exercised only on a broken build
+    raise ImportError(
+        "cholespy_amd native extension is not built; run `python cholespy_amd/build.py` "
+        f"({exc})") from exc
+
+MatrixType = _core.MatrixType
+CholeskySolverF = _core.CholeskySolverF
+CholeskySolverD = _core.CholeskySolverD
+__version__ = _core.__version__
+
+__all__ = ["MatrixType", "CholeskySolverF", "CholeskySolverD", "__version__"]

@@ -1,0 +1,128 @@
+// Shared device helpers for the gfx950 solve-phase kernels.
+//
+// Cross-workgroup hand-off. The reference publishes a solved row through a
+// separate `volatile bool` flag after a __threadfence (kernels/kernels.cu:
+// 106-111, 122-130). Here every published value is its own flag: buffers
+// that rows wait on are pre-filled with a signalling-NaN sentinel, a
+// producer writes each value with one agent-scope relaxed atomic store, and
+// a consumer spins with agent-scope relaxed atomic loads until the word is
+// no longer the sentinel. Single-copy atomicity of aligned 32/64-bit
+// accesses makes a non-sentinel word the final value, so no fences (and no
+// L2 write-back / invalidate) are needed on gfx950; only the `sc1` scope bit
+// on the accesses themselves. Arithmetic never produces a signalling NaN,
+// so a computed result can never be mistaken for the sentinel.
+//
+// Bounded waits. Every spin loop counts its polls; past a limit it records
+// CHOLESPY_ERR_TIMEOUT in the status word, raises the abort word and stops
+// waiting, and every other spinning lane sees the abort word and stops too,
+// so a broken schedule ends the kernel instead of hanging the device.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cholespy_amd {
+
+constexpr int kWave = 64;
+constexpr int kMaxRhs = 128;
+constexpr int kStatusTimeout = 4;   // matches CHOLESPY_ERR_TIMEOUT
+
+template <typename F> struct Bits;
+template <> struct Bits<float> {
+    using U = uint32_t;
+    static constexpr U sentinel = 0x7FA5A5A5u;   // signalling NaN
+};
+template <> struct Bits<double> {
+    using U = uint64_t;
+    static constexpr U sentinel = 0x7FF4A5A5A5A5A5A5ull;   // signalling NaN
+};
+
+template <typename F>
+__device__ __forceinline__ typename Bits<F>::U as_bits(F v) {
+    return __builtin_bit_cast(typename Bits<F>::U, v);
+}
+template <typename F>
+__device__ __forceinline__ F from_bits(typename Bits<F>::U u) {
+    return __builtin_bit_cast(F, u);
+}
+
+// Agent-scope relaxed store of one value (its own publication).
+template <typename F>
+__device__ __forceinline__ void publish(F *p, F v) {
+    using U = typename Bits<F>::U;
+    __hip_atomic_store(reinterpret_cast<U *>(p), as_bits(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename F>
+__device__ __forceinline__ typename Bits<F>::U load_bits(const F *p) {
+    using U = typename Bits<F>::U;
+    return __hip_atomic_load(reinterpret_cast<const U *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ int load_int(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_int(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// status[0] = error code (first writer wins), status[1] = abort flag.
+__device__ __forceinline__ void raise_timeout(int *status) {
+    int expected = 0;
+    __hip_atomic_compare_exchange_strong(status, &expected, kStatusTimeout, __ATOMIC_RELAXED,
+                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    store_int(status + 1, 1);
+}
+
+constexpr uint32_t kSpinLimit = 1u << 23;
+
+// Waits until *p is no longer the sentinel and returns the value. On
+// timeout or abort returns 0 (the kernel's result is then invalid and the
+// status word says so).
+template <typename F>
+__device__ __forceinline__ F wait_value(const F *p, int *status) {
+    using U = typename Bits<F>::U;
+    U u = load_bits(p);
+    if (u != Bits<F>::sentinel) return from_bits<F>(u);
+    uint32_t spins = 0;
+    while (true) {
+        __builtin_amdgcn_s_sleep(1);
+        u = load_bits(p);
+        if (u != Bits<F>::sentinel) return from_bits<F>(u);
+        if ((++spins & 255u) == 0) {
+            if (load_int(status + 1)) return F(0);
+            if (spins >= kSpinLimit) {
+                raise_timeout(status);
+                return F(0);
+            }
+        }
+    }
+}
+
+// Waits until *p is non-zero and returns it (same watchdog as wait_value).
+__device__ __forceinline__ int wait_nonzero(const int *p, int *status) {
+    int v = load_int(p);
+    if (v != 0) return v;
+    uint32_t spins = 0;
+    while (true) {
+        __builtin_amdgcn_s_sleep(1);
+        v = load_int(p);
+        if (v != 0) return v;
+        if ((++spins & 255u) == 0) {
+            if (load_int(status + 1)) return 1;
+            if (spins >= kSpinLimit) {
+                raise_timeout(status);
+                return 1;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+}  // namespace cholespy_amd

@@ -1,0 +1,319 @@
+// Python module cholespy_amd._cholespy_core: MatrixType, CholeskySolverF,
+// CholeskySolverD and __version__, with the reference's argument names,
+// validation order, error types and messages (src/main.cpp:16-136,
+// src/docstr.h:1-48).
+//
+// nanobind is not available, so arrays are taken through the DLPack protocol
+// (__dlpack__) directly: NumPy, PyTorch CPU and PyTorch ROCm tensors work
+// unchanged. Constructor arrays may be any integer / float dtype and any
+// stride (the reference's implicit conversion of `ii`, `jj`, `x`, e.g. int64
+// index arrays); solve() takes b and x without conversion (src/main.cpp:108-109).
+// A ROCm tensor reports DLPack device type 10 (kDLROCM); it is accepted
+// where the reference accepts CUDA (2).
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "cholesky_solver.hpp"
+#include "dlpack_abi.hpp"
+#include "docstrings.hpp"
+
+namespace py = pybind11;
+using namespace cholespy_amd;
+
+namespace {
+
+enum class Dev { CPU, GPU, Other };
+
+// A borrowed view of an array exported through DLPack. Calls the producer's
+// deleter when it goes out of scope.
+class DLView {
+public:
+    explicit DLView(py::handle obj) {
+        if (!py::hasattr(obj, "__dlpack__"))
+            throw py::type_error("Expected an array supporting the DLPack protocol (__dlpack__).");
+        py::object cap;
+        bool versioned = false;
+        try {
+            cap = obj.attr("__dlpack__")();
+        } catch (py::error_already_set &e) {
+            // NumPy refuses legacy export of read-only arrays; DLPack 1.0 has a
+            // read-only flag and is allowed.
+            if (!e.matches(PyExc_BufferError)) throw;
+            cap = obj.attr("__dlpack__")(py::arg("max_version") = py::make_tuple(1, 0));
+            versioned = true;
+        }
+        PyObject *c = cap.ptr();
+        if (PyCapsule_IsValid(c, "dltensor_versioned")) versioned = true;
+        if (versioned) {
+            vmt_ = static_cast<dl::ManagedTensorVersioned *>(PyCapsule_GetPointer(c, "dltensor_versioned"));
+            if (!vmt_) throw py::error_already_set();
+            PyCapsule_SetName(c, "used_dltensor_versioned");
+            t_ = &vmt_->dl_tensor;
+            readonly_ = (vmt_->flags & 1u) != 0;
+        } else {
+            mt_ = static_cast<dl::ManagedTensor *>(PyCapsule_GetPointer(c, "dltensor"));
+            if (!mt_) throw py::error_already_set();
+            PyCapsule_SetName(c, "used_dltensor");
+            t_ = &mt_->dl_tensor;
+        }
+    }
+    ~DLView() {
+        if (mt_ && mt_->deleter) mt_->deleter(mt_);
+        if (vmt_ && vmt_->deleter) vmt_->deleter(vmt_);
+    }
+    DLView(const DLView &) = delete;
+    DLView &operator=(const DLView &) = delete;
+
+    Dev device() const {
+        const int dt = t_->device.device_type;
+        if (dt == dl::kCPU) return Dev::CPU;
+        if (dt == dl::kCUDA || dt == dl::kROCM) return Dev::GPU;
+        return Dev::Other;
+    }
+    int device_type() const { return t_->device.device_type; }
+    int ndim() const { return t_->ndim; }
+    int64_t shape(int i) const { return t_->shape[i]; }
+    int64_t stride(int i) const { return t_->strides ? t_->strides[i] : contiguous_stride(i); }
+    void *data() const { return static_cast<char *>(t_->data) + t_->byte_offset; }
+    const dl::DataType &dtype() const { return t_->dtype; }
+    bool readonly() const { return readonly_; }
+    bool c_contig() const {
+        if (!t_->strides) return true;
+        int64_t expect = 1;
+        for (int i = t_->ndim - 1; i >= 0; --i) {
+            if (t_->shape[i] != 1 && t_->strides[i] != expect) return false;
+            expect *= t_->shape[i];
+        }
+        return true;
+    }
+    int64_t size() const {
+        int64_t s = 1;
+        for (int i = 0; i < t_->ndim; ++i) s *= t_->shape[i];
+        return s;
+    }
+
+private:
+    int64_t contiguous_stride(int i) const {
+        int64_t s = 1;
+        for (int j = t_->ndim - 1; j > i; --j) s *= t_->shape[j];
+        return s;
+    }
+    dl::ManagedTensor *mt_ = nullptr;
+    dl::ManagedTensorVersioned *vmt_ = nullptr;
+    dl::Tensor *t_ = nullptr;
+    bool readonly_ = false;
+};
+
+// Ensures obj exposes __dlpack__; plain sequences go through numpy.asarray.
+py::object as_exportable(py::handle obj) {
+    if (py::hasattr(obj, "__dlpack__")) return py::reinterpret_borrow<py::object>(obj);
+    return py::module_::import("numpy").attr("asarray")(obj);
+}
+
+template <typename T>
+T read_elem(const char *p, const dl::DataType &dt) {
+    switch (dt.code) {
+    case dl::kInt:
+        switch (dt.bits) {
+        case 8: { int8_t v; std::memcpy(&v, p, 1); return (T)v; }
+        case 16: { int16_t v; std::memcpy(&v, p, 2); return (T)v; }
+        case 32: { int32_t v; std::memcpy(&v, p, 4); return (T)v; }
+        case 64: { int64_t v; std::memcpy(&v, p, 8); return (T)v; }
+        }
+        break;
+    case dl::kUInt:
+        switch (dt.bits) {
+        case 8: { uint8_t v; std::memcpy(&v, p, 1); return (T)v; }
+        case 16: { uint16_t v; std::memcpy(&v, p, 2); return (T)v; }
+        case 32: { uint32_t v; std::memcpy(&v, p, 4); return (T)v; }
+        case 64: { uint64_t v; std::memcpy(&v, p, 8); return (T)v; }
+        }
+        break;
+    case dl::kFloat:
+        if (std::is_integral<T>::value) break;   // no implicit float -> index conversion
+        switch (dt.bits) {
+        case 32: { float v; std::memcpy(&v, p, 4); return (T)v; }
+        case 64: { double v; std::memcpy(&v, p, 8); return (T)v; }
+        }
+        break;
+    }
+    throw py::type_error("Unsupported array dtype for this argument.");
+}
+
+// Copies a 1-D array (CPU or GPU, any stride) into a host vector of T,
+// converting the element type: the constructor's implicit conversion.
+template <typename T>
+std::vector<T> to_host(const DLView &v) {
+    const int64_t n = v.shape(0);
+    const int64_t st = v.stride(0);
+    const int eb = v.dtype().bits / 8;
+    if (v.dtype().lanes != 1 || eb == 0) throw py::type_error("Unsupported array dtype.");
+    std::vector<T> out((size_t)n);
+    if (n == 0) return out;
+    const char *base = static_cast<const char *>(v.data());
+    std::vector<char> staged;
+    if (v.device() == Dev::GPU) {
+        const int64_t lo = st >= 0 ? 0 : (n - 1) * st;
+        const int64_t span = ((n - 1) * (st >= 0 ? st : -st) + 1) * eb;
+        staged.resize((size_t)span);
+        throw_on_error(cholespy_copy_to_host(staged.data(), base + lo * eb, (size_t)span));
+        base = staged.data() - lo * eb;
+    }
+    for (int64_t i = 0; i < n; ++i) out[(size_t)i] = read_elem<T>(base + i * st * eb, v.dtype());
+    return out;
+}
+
+template <typename Float>
+void declare_solver(py::module_ &m, const char *name, const char *doc) {
+    using Class = CholeskySolver<Float>;
+    py::class_<Class>(m, name, doc)
+        .def(py::init([](uint32_t n_rows, py::object ii_o, py::object jj_o, py::object x_o,
+                         MatrixType type) {
+                 ii_o = as_exportable(ii_o);
+                 jj_o = as_exportable(jj_o);
+                 x_o = as_exportable(x_o);
+                 DLView ii(ii_o), jj(jj_o), x(x_o);
+                 if (ii.ndim() != 1 || jj.ndim() != 1 || x.ndim() != 1)
+                     throw py::type_error("ii, jj and x must be one-dimensional arrays.");
+                 // src/main.cpp:28-43
+                 if (type == MatrixType::COO) {
+                     if (ii.shape(0) != jj.shape(0))
+                         throw std::invalid_argument("Sparse COO matrix: the two index arrays should have the same size.");
+                     if (ii.shape(0) != x.shape(0))
+                         throw std::invalid_argument("Sparse COO matrix: the index and data arrays should have the same size.");
+                 } else if (type == MatrixType::CSR) {
+                     if (jj.shape(0) != x.shape(0))
+                         throw std::invalid_argument("Sparse CSR matrix: the column index and data arrays should have the same size.");
+                     if (ii.shape(0) != (int64_t)n_rows + 1)
+                         throw std::invalid_argument("Sparse CSR matrix: Invalid size for row pointer array.");
+                 } else {
+                     if (jj.shape(0) != x.shape(0))
+                         throw std::invalid_argument("Sparse CSC matrix: the row index and data arrays should have the same size.");
+                     if (ii.shape(0) != (int64_t)n_rows + 1)
+                         throw std::invalid_argument("Sparse CSC matrix: Invalid size for column pointer array.");
+                 }
+                 // src/main.cpp:44-45
+                 if (ii.device_type() != jj.device_type() || ii.device_type() != x.device_type())
+                     throw std::invalid_argument("All input tensors should be on the same device!");
+                 if (n_rows > 0x7fffffffu) throw std::invalid_argument("n_rows does not fit in 32 bits.");
+                 const Dev dev = ii.device();
+                 if (dev != Dev::CPU && dev != Dev::GPU)
+                     throw std::invalid_argument("Unsupported input device! Only CPU and CUDA arrays are supported.");
+                 if (dev == Dev::GPU) throw_on_error(cholespy_init());   // src/main.cpp:51
+                 std::vector<int> iv = to_host<int>(ii), jv = to_host<int>(jj);
+                 std::vector<double> xv = to_host<double>(x);
+                 py::gil_scoped_release release;
+                 return std::make_unique<Class>((int)n_rows, (int64_t)xv.size(), iv.data(), jv.data(),
+                                                xv.data(), type, dev == Dev::CPU);
+             }),
+             py::arg("n_rows"), py::arg("ii"), py::arg("jj"), py::arg("x"), py::arg("type"),
+             doc_constructor)
+        .def(
+            "solve",
+            [](Class &self, py::object b_o, py::object x_o) {
+                DLView b(b_o), x(x_o);
+                const bool is_f32 = std::is_same<Float, float>::value;
+                const auto good = [&](const DLView &v) {
+                    return v.dtype().code == dl::kFloat && v.dtype().bits == (is_f32 ? 32 : 64) &&
+                           v.dtype().lanes == 1 && v.c_contig();
+                };
+                // noconvert (src/main.cpp:108-109): wrong dtype or layout is a type error.
+                if (!good(b) || !good(x))
+                    throw py::type_error(std::string("solve(): b and x must be C-contiguous ") +
+                                         (is_f32 ? "float32" : "float64") +
+                                         " arrays (no implicit conversion, x is written in place).");
+                if (x.readonly()) throw py::type_error("solve(): x must be writable.");
+                // src/main.cpp:83-88
+                if (b.ndim() != 1 && b.ndim() != 2)
+                    throw std::invalid_argument("Expected 1D or 2D tensors as input.");
+                if (b.ndim() != x.ndim() || b.shape(0) != x.shape(0) ||
+                    (b.ndim() == 2 && b.shape(1) != x.shape(1)))
+                    throw std::invalid_argument("x and b should have the same dimensions.");
+                if (b.device() != x.device())
+                    throw std::invalid_argument("x and b should be on the same device.");
+                const int n_rhs = b.ndim() == 2 ? (int)b.shape(1) : 1;
+                // src/main.cpp:91-106
+                if (b.device() == Dev::CPU) {
+                    if (!self.is_cpu())
+                        throw std::invalid_argument("Input device is CPU but the solver was initialized for CUDA.");
+                } else if (b.device() == Dev::GPU) {
+                    if (self.is_cpu())
+                        throw std::invalid_argument("Input device is CUDA but the solver was initialized for CPU.");
+                } else {
+                    throw std::invalid_argument("Unsupported input device! Only CPU and CUDA arrays are supported.");
+                }
+                // Not checked by the reference, whose kernels would then read
+                // and write out of bounds.
+                if (b.shape(0) != self.n())
+                    throw std::invalid_argument("b and x should have n_rows rows.");
+                if (b.device() == Dev::CPU) {
+                    const Float *bp = static_cast<const Float *>(b.data());
+                    Float *xp = static_cast<Float *>(x.data());
+                    py::gil_scoped_release release;
+                    self.solve_cpu(n_rhs, bp, xp);
+                } else {
+                    self.solve_cuda(n_rhs, b.data(), x.data());
+                }
+            },
+            py::arg("b"), py::arg("x"), doc_solve)
+        // Inspection hooks (not part of the reference API): the host factor,
+        // the GPU level analysis and the device watchdog status.
+        .def("_factor",
+             [](const Class &self) {
+                 const Factor &f = self.factor();
+                 py::dict d;
+                 d["perm"] = py::array_t<int>(f.perm.size(), f.perm.data());
+                 d["colptr"] = py::array_t<int>(f.colptr.size(), f.colptr.data());
+                 d["rowidx"] = py::array_t<int>(f.rowidx.size(), f.rowidx.data());
+                 d["val"] = py::array_t<double>(f.val.size(), f.val.data());
+                 d["parent"] = py::array_t<int>(f.parent.size(), f.parent.data());
+                 return d;
+             })
+        .def("_matrix",
+             [](const Class &self) {
+                 const LowerCSC &a = self.matrix();
+                 py::dict d;
+                 d["colptr"] = py::array_t<int>(a.colptr.size(), a.colptr.data());
+                 d["rowidx"] = py::array_t<int>(a.rowidx.size(), a.rowidx.data());
+                 d["val"] = py::array_t<double>(a.val.size(), a.val.data());
+                 return d;
+             })
+        .def("_levels",
+             [](const Class &self, bool lower) {
+                 std::vector<int> order, row_level;
+                 self.levels(lower, order, row_level);
+                 return py::make_tuple(py::array_t<int>(order.size(), order.data()),
+                                       py::array_t<int>(row_level.size(), row_level.data()));
+             },
+             py::arg("lower"))
+        .def("_n_levels", [](const Class &self, bool lower) { return self.n_levels(lower); },
+             py::arg("lower"))
+        .def("_check", [](const Class &self) { self.check(); })
+        .def_property_readonly("_n", [](const Class &self) { return self.n(); })
+        .def_property_readonly("_is_cpu", [](const Class &self) { return self.is_cpu(); });
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_cholespy_core, m) {
+    py::enum_<MatrixType>(m, "MatrixType", doc_matrix_type)
+        .value("CSC", MatrixType::CSC)
+        .value("CSR", MatrixType::CSR)
+        .value("COO", MatrixType::COO);
+
+    declare_solver<float>(m, "CholeskySolverF", doc_cholesky_f);
+    declare_solver<double>(m, "CholeskySolverD", doc_cholesky_d);
+
+    // Release library state when the module is unloaded (src/main.cpp:126-129).
+    m.add_object("_shutdown_guard", py::capsule(reinterpret_cast<void *>(1), [](void *) {
+                     cholespy_shutdown();
+                 }));
+    m.attr("__version__") = "2.2.2";
+}

@@ -1,0 +1,85 @@
+#include "cholesky_solver.hpp"
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+
+namespace cholespy_amd {
+
+void throw_on_error(int status) {
+    if (status == CHOLESPY_OK) return;
+    const std::string msg = cholespy_last_error();
+    if (status == CHOLESPY_ERR_INVALID_ARGUMENT) throw std::invalid_argument(msg);
+    throw std::runtime_error(msg);
+}
+
+template <typename Float>
+CholeskySolver<Float>::CholeskySolver(int n_rows, int64_t nnz, const int *ii, const int *jj,
+                                      const double *x, MatrixType type, bool cpu)
+    : n_(n_rows), cpu_(cpu) {
+    A_ = normalize_input(n_rows, nnz, ii, jj, x, type);
+    factor_ = cholesky_factor(A_);
+    if (cpu_) return;
+
+    // GPU tail of the reference's factorize() (src/cholesky_solver.cpp:244-297):
+    // upload perm and L, then analyse L and L'.
+    throw_on_error(cholespy_solver_create(n_, (int)sizeof(Float), &gpu_));
+    try {
+        throw_on_error(cholespy_solver_upload_factor(gpu_, factor_.perm.data(), factor_.colptr.data(),
+                                                     factor_.rowidx.data(), factor_.val.data(),
+                                                     factor_.nnz()));
+        throw_on_error(cholespy_solver_analyze(gpu_, 1, &n_levels_lower_));
+        throw_on_error(cholespy_solver_analyze(gpu_, 0, &n_levels_upper_));
+    } catch (...) {
+        cholespy_solver_destroy(gpu_);
+        gpu_ = nullptr;
+        throw;
+    }
+}
+
+template <typename Float>
+CholeskySolver<Float>::~CholeskySolver() {
+    if (gpu_) cholespy_solver_destroy(gpu_);
+}
+
+template <typename Float>
+void CholeskySolver<Float>::solve_cuda(int n_rhs, const void *b, void *x) {
+    if (n_rhs > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
+    if (std::is_same<Float, float>::value)
+        throw_on_error(cholespy_solver_solve_f32(gpu_, n_rhs, static_cast<const float *>(b),
+                                                 static_cast<float *>(x), nullptr));
+    else
+        throw_on_error(cholespy_solver_solve_f64(gpu_, n_rhs, static_cast<const double *>(b),
+                                                 static_cast<double *>(x), nullptr));
+}
+
+template <typename Float>
+void CholeskySolver<Float>::solve_cpu(int n_rhs, const Float *b, Float *x) {
+    if (n_rhs <= 0 || n_ == 0) return;
+    // Row-major Float -> column-major FP64 (src/cholesky_solver.cpp:449-453).
+    std::vector<double> buf((size_t)n_ * (size_t)n_rhs);
+    for (int i = 0; i < n_; ++i)
+        for (int j = 0; j < n_rhs; ++j) buf[(size_t)j * n_ + i] = (double)b[(size_t)i * n_rhs + j];
+    factor_solve_colmajor(factor_, n_rhs, buf.data());
+    for (int i = 0; i < n_; ++i)
+        for (int j = 0; j < n_rhs; ++j) x[(size_t)i * n_rhs + j] = (Float)buf[(size_t)j * n_ + i];
+}
+
+template <typename Float>
+void CholeskySolver<Float>::levels(bool lower, std::vector<int> &order,
+                                   std::vector<int> &row_level) const {
+    if (!gpu_) throw std::invalid_argument("Level analysis only exists for GPU solvers.");
+    order.resize((size_t)n_);
+    row_level.resize((size_t)n_);
+    throw_on_error(cholespy_solver_get_levels(gpu_, lower ? 1 : 0, order.data(), row_level.data()));
+}
+
+template <typename Float>
+void CholeskySolver<Float>::check() const {
+    if (gpu_) throw_on_error(cholespy_solver_check(gpu_));
+}
+
+template class CholeskySolver<float>;
+template class CholeskySolver<double>;
+
+}  // namespace cholespy_amd

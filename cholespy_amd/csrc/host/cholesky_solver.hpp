@@ -1,0 +1,56 @@
+// CholeskySolver<Float>: the host object behind CholeskySolverF / D.
+// Mirrors src/cholesky_solver.h:22-99: the constructor normalises the input
+// and factors on the host; in GPU mode the factor goes through the C ABI of
+// libcholespy_amd (include/cholespy_amd.h) for the level analysis and the
+// triangular solves; in CPU mode solves stay on the host in FP64.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+#include "../../../include/cholespy_amd.h"
+#include "factor.hpp"
+#include "sparse_input.hpp"
+
+namespace cholespy_amd {
+
+template <typename Float>
+class CholeskySolver {
+public:
+    // src/cholesky_solver.h:36 / src/cholesky_solver.cpp:151-198
+    CholeskySolver(int n_rows, int64_t nnz, const int *ii, const int *jj, const double *x,
+                   MatrixType type, bool cpu);
+    ~CholeskySolver();
+    CholeskySolver(const CholeskySolver &) = delete;
+    CholeskySolver &operator=(const CholeskySolver &) = delete;
+
+    // src/cholesky_solver.cpp:415-432. b and x are device addresses of
+    // row-major (n x n_rhs) arrays; asynchronous on the null stream.
+    void solve_cuda(int n_rhs, const void *b, void *x);
+
+    // src/cholesky_solver.cpp:434-463: row-major Float in/out, FP64 inside.
+    void solve_cpu(int n_rhs, const Float *b, Float *x);
+
+    bool is_cpu() const { return cpu_; }
+    int n() const { return n_; }
+    const Factor &factor() const { return factor_; }
+    const LowerCSC &matrix() const { return A_; }
+    int n_levels(bool lower) const { return lower ? n_levels_lower_ : n_levels_upper_; }
+    // Level-sorted rows and per-row levels as produced by the GPU analysis.
+    void levels(bool lower, std::vector<int> &order, std::vector<int> &row_level) const;
+    // Waits for the device and raises if a kernel reported a timeout.
+    void check() const;
+
+private:
+    int n_ = 0;
+    bool cpu_ = true;
+    LowerCSC A_;
+    Factor factor_;
+    cholespy_gpu_solver *gpu_ = nullptr;
+    int n_levels_lower_ = 0, n_levels_upper_ = 0;
+};
+
+// Raises the C++ exception matching a C ABI status (invalid argument ->
+// std::invalid_argument, anything else -> std::runtime_error).
+void throw_on_error(int status);
+
+}  // namespace cholespy_amd

@@ -1,0 +1,123 @@
+/*
+ * cholespy_amd C ABI: the device side of cholespy's GPU solve phase on
+ * MI355X (gfx950), as one shared library (libcholespy_amd.so) holding the
+ * hand-written HIP kernels and the entry points below.
+ *
+ * It replaces, in the reference:
+ *   src/cuda_driver.h:16-61 / src/cuda_driver.cpp:58-149  (dlopen of libcuda,
+ *       PTX module loading, 6 CUfunction handles, context push/pop), and
+ *   the device half of CholeskySolver<Float> (src/cholesky_solver.h:60-98,
+ *       src/cholesky_solver.cpp:180-188, 244-432, 465-486),
+ * with kernels compiled into the library by hipcc --offload-arch=gfx950 (no
+ * run-time module loading). Only plain C types cross this boundary: device
+ * buffers are passed as pointers, streams as an opaque hipStream_t value
+ * (NULL = the null stream, which is what the reference launches on).
+ *
+ * Conventions
+ *   - Every function returns a cholespy_status; on failure a message is
+ *     available from cholespy_last_error() (per calling thread).
+ *   - A solver handle owns every device buffer it allocates. b is read only;
+ *     x is overwritten. Calls on one handle must not overlap (the handle's
+ *     scratch buffers are shared by its solves, as in the reference).
+ *   - Dense right-hand sides are row-major (n x k), element (i, j) at
+ *     [i * k + j], exactly as kernels/kernels.cu:87,114,120 index them.
+ */
+#ifndef CHOLESPY_AMD_H
+#define CHOLESPY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum cholespy_status {
+    CHOLESPY_OK = 0,
+    CHOLESPY_ERR_INVALID_ARGUMENT = 1, /* bad sizes, null pointers, k > 128 */
+    CHOLESPY_ERR_HIP = 2,              /* a HIP runtime call failed */
+    CHOLESPY_ERR_NO_DEVICE = 3,        /* no GPU visible */
+    CHOLESPY_ERR_TIMEOUT = 4,          /* a kernel's bounded wait expired */
+    CHOLESPY_ERR_STATE = 5             /* call out of order (e.g. solve before analyze) */
+} cholespy_status;
+
+typedef struct cholespy_gpu_solver cholespy_gpu_solver; /* opaque */
+
+/* Lazy, idempotent device initialisation. Replaces init_cuda()
+ * (src/cuda_driver.cpp:58-136). Selects the calling thread's current HIP
+ * device and records its compute-unit count for persistent launches. */
+int cholespy_init(void);
+
+/* Releases library-wide state. Replaces shutdown_cuda()
+ * (src/cuda_driver.cpp:138-149); called when the Python module unloads
+ * (src/main.cpp:127-129). Safe to call more than once. */
+int cholespy_shutdown(void);
+
+/* Message describing the last failure on this thread ("" if none).
+ * Replaces cuda_check_impl()'s stderr print (src/cuda_driver.cpp:48-56). */
+const char *cholespy_last_error(void);
+
+/* Number of visible GPUs (0 when none); does not initialise a context. */
+int cholespy_device_count(int *count);
+
+/* Synchronous device-to-host copy, used by the binding for GPU inputs of the
+ * constructor (src/main.cpp:55-61). */
+int cholespy_copy_to_host(void *dst, const void *device_src, size_t bytes);
+
+/* Creates a solver for an n x n factor whose device arithmetic uses
+ * float_bytes = 4 (float, CholeskySolverF) or 8 (double, CholeskySolverD).
+ * Replaces the GPU allocations of the constructor
+ * (src/cholesky_solver.cpp:180-188). */
+int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out);
+
+/* Frees every device buffer of the handle. Replaces ~CholeskySolver
+ * (src/cholesky_solver.cpp:465-486). NULL is accepted. */
+int cholespy_solver_destroy(cholespy_gpu_solver *solver);
+
+/* Uploads the factor L (L L' = P A P', perm[k] = row of A placed at k) given
+ * in compressed-column form with rows ascending and the diagonal first in
+ * each column, values in double. The library derives the row-compressed
+ * copy of L (diagonal last) for the forward pass and uses the column form as
+ * the row-compressed L' for the backward pass, casting values to the
+ * solver's precision. Replaces the GPU tail of factorize() and the uploads
+ * of analyze_cuda() (src/cholesky_solver.cpp:244-288, 300-313). */
+int cholespy_solver_upload_factor(cholespy_gpu_solver *solver, const int *perm,
+                                  const int *colptr, const int *rowidx,
+                                  const double *val, int64_t nnz);
+
+/* Level-set analysis of L (lower = 1) or L' (lower = 0) on the GPU followed
+ * by the stable counting sort of rows by level. Replaces analyze_cuda()
+ * (src/cholesky_solver.cpp:299-376) and the kernels analysis_lower /
+ * analysis_upper (kernels/kernels.cu:8-60). Writes the number of levels
+ * (max level + 1) to *n_levels when it is not NULL. */
+int cholespy_solver_analyze(cholespy_gpu_solver *solver, int lower, int *n_levels);
+
+/* Copies the analysis results to the host: levels[n] = rows sorted by level
+ * (stable in row index; the `levels` array of analyze_cuda) and, when not
+ * NULL, row_level[n] = level of each row. */
+int cholespy_solver_get_levels(const cholespy_gpu_solver *solver, int lower, int *levels,
+                               int *row_level);
+
+/* x := A^{-1} b for n_rhs right-hand sides (1 <= n_rhs <= 128), b and x
+ * device pointers to row-major (n x n_rhs) arrays of the solver's
+ * precision. Asynchronous on `stream`. Replaces solve_cuda() and
+ * launch_kernel() (src/cholesky_solver.cpp:378-432) and the kernels
+ * solve_{lower,upper}_{float,double} (kernels/kernels.cu:64-218). Returns
+ * CHOLESPY_ERR_INVALID_ARGUMENT for n_rhs > 128, as the reference throws
+ * "The number of RHS should be less than 128." (src/cholesky_solver.cpp:419). */
+int cholespy_solver_solve_f32(cholespy_gpu_solver *solver, int n_rhs, const float *b,
+                              float *x, void *stream);
+int cholespy_solver_solve_f64(cholespy_gpu_solver *solver, int n_rhs, const double *b,
+                              double *x, void *stream);
+
+/* Reads and clears the handle's device status word (the bounded-wait
+ * watchdog of the solve and analysis kernels). Synchronises the device.
+ * Returns CHOLESPY_ERR_TIMEOUT if a kernel gave up waiting since the last
+ * call, CHOLESPY_OK otherwise. */
+int cholespy_solver_check(cholespy_gpu_solver *solver);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CHOLESPY_AMD_H */
