@@ -7,6 +7,16 @@ there is no Python fallback.
 """
 from importlib import import_module as _import_module
 
+# PyTorch-ROCm bundles its own HIP runtime under the same SONAME
+# (libamdhip64.so.7) as the system ROCm one this extension links. Only the
+# first one loaded is used by the whole process, and a process that mixes
+# them sees no device; loading torch first makes its runtime the one both
+# use. Without torch the system runtime is loaded.
+try:  # pragma: no cover - depends on the environment
+    import torch as _torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    _torch = None
+
 try:
     _core = _import_module("cholespy_amd._cholespy_core")
 except ImportError as exc:  # pragma: no cover - exercised only on a broken build

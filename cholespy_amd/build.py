@@ -33,7 +33,8 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare"]
 
-DEVICE_SOURCES = ["analysis.hip", "trsv_rows.hip", "capi.hip"]
+DEVICE_SOURCES = ["analysis.hip", "trsv_rows.hip", "trsv_blocks.hip", "refine.hip", "capi.hip"]
+PLAN_SOURCES = ["block_plan.cpp"]   # host code linked into libcholespy_amd.so
 HOST_SOURCES = ["sparse_input.cpp", "ordering.cpp", "factor.cpp", "cholesky_solver.cpp",
                 "bindings.cpp"]
 
@@ -63,6 +64,9 @@ def build(verbose: bool = False) -> dict:
     for src in DEVICE_SOURCES:
         obj = OBJ / (Path(src).stem + ".hip.o")
         jobs.append([HIPCC, *HIP_FLAGS, "-c", str(CSRC / "device" / src), "-o", str(obj)])
+    for src in PLAN_SOURCES:
+        obj = OBJ / (Path(src).stem + ".plan.o")
+        jobs.append([CXX, *HOST_FLAGS, "-c", str(CSRC / "device" / src), "-o", str(obj)])
     for src in HOST_SOURCES:
         obj = OBJ / (Path(src).stem + ".o")
         extra = _pybind_includes() + ["-fvisibility=hidden"] if src == "bindings.cpp" else []
@@ -75,6 +79,7 @@ def build(verbose: bool = False) -> dict:
 
     lib = PKG / LIB_NAME
     dev_objs = [str(OBJ / (Path(s).stem + ".hip.o")) for s in DEVICE_SOURCES]
+    dev_objs += [str(OBJ / (Path(s).stem + ".plan.o")) for s in PLAN_SOURCES]
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-fPIC", *dev_objs, "-o", str(lib)])
 
     ext = PKG / EXT_NAME

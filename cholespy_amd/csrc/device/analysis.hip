@@ -11,9 +11,9 @@
 //
 // MI355X design: one wavefront per row instead of one thread, so a row's
 // dependencies are read 64 at a time (coalesced) and reduced with a
-// wave-wide max; rows are assigned statically to the wavefronts of a
-// co-resident persistent grid in dependency order, so the wavefront owning a
-// row's dependency always runs ahead of it. The per-row "analyzed" flag and
+// wave-wide max; wavefronts of a persistent grid take chunks of rows in
+// dependency order from the ticket queues of common.hpp (the reference takes
+// one atomicAdd ticket per row, kernels/kernels.cu:9). The per-row "analyzed" flag and
 // level of the reference (two arrays, a fence in between) collapse into one
 // word lvl1[r] = level(r) + 1 (0 = not yet known), published with one
 // agent-scope atomic store.
@@ -25,19 +25,28 @@ namespace cholespy_amd {
 template <bool LOWER>
 __global__ __launch_bounds__(kRowBlock) void level_analysis(int n, const int *__restrict__ rowptr,
                                                             const int *__restrict__ cols,
-                                                            int *lvl1, int *status) {
+                                                            int *lvl1, int *status, QueueArgs qa) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int waves = gridDim.x * (kRowBlock / kWave);
     const int w = blockIdx.x * (kRowBlock / kWave) + (threadIdx.x / kWave);
-    for (int t = w; t < n; t += waves) {
-        const int row = LOWER ? t : n - 1 - t;
-        const int rs = rowptr[row], re = rowptr[row + 1];
-        const int lo = LOWER ? rs : rs + 1;
-        const int hi = LOWER ? re - 1 : re;
-        int lv = 0;
-        for (int i = lo + lane; i < hi; i += kWave) lv = max(lv, wait_nonzero(&lvl1[cols[i]], status));
-        lv = wave_max(lv);
-        if (lane == 0) store_int(&lvl1[row], lv + 1);
+    const int q = w % qa.nq;
+    const int nchunks = (n + kRowChunk - 1) / kRowChunk;
+    int next = lane == 0 ? take_ticket(qa, q, nchunks) : 0;
+    next = __shfl(next, 0, kWave);
+    while (next >= 0) {
+        const int chunk = next;
+        int pre = lane == 0 ? take_ticket(qa, q, nchunks) : 0;   // in flight during the chunk
+        const int t_end = min(n, (chunk + 1) * kRowChunk);
+        for (int t = chunk * kRowChunk; t < t_end; ++t) {
+            const int row = LOWER ? t : n - 1 - t;
+            const int rs = rowptr[row], re = rowptr[row + 1];
+            const int lo = LOWER ? rs : rs + 1;
+            const int hi = LOWER ? re - 1 : re;
+            int lv = 0;
+            for (int i = lo + lane; i < hi; i += kWave) lv = max(lv, wait_nonzero(&lvl1[cols[i]], status));
+            lv = wave_max(lv);
+            if (lane == 0) store_int(&lvl1[row], lv + 1);
+        }
+        next = __shfl(pre, 0, kWave);
     }
 }
 
@@ -47,10 +56,25 @@ const void *level_analysis_kernel(bool lower) {
 }
 
 hipError_t launch_level_analysis(bool lower, int n, const int *rowptr, const int *cols, int *lvl1,
-                                 int *status, int grid, hipStream_t stream) {
-    void *args[] = {&n, &rowptr, &cols, &lvl1, &status};
-    return hipLaunchCooperativeKernel(level_analysis_kernel(lower), dim3(grid), dim3(kRowBlock),
-                                      args, 0, stream);
+                                 int *status, int grid, QueueState &queues, hipStream_t stream) {
+    QueueArgs qa;
+    queues.next_launch(qa, grid * (kRowBlock / kWave), (n + kRowChunk - 1) / kRowChunk);
+    if (lower)
+        level_analysis<true><<<grid, kRowBlock, 0, stream>>>(n, rowptr, cols, lvl1, status, qa);
+    else
+        level_analysis<false><<<grid, kRowBlock, 0, stream>>>(n, rowptr, cols, lvl1, status, qa);
+    return hipGetLastError();
+}
+
+void QueueState::next_launch(QueueArgs &qa, int workers, int tickets) {
+    qa.ctr = ctr;
+    qa.nq = nq;
+    for (int q = 0; q < kMaxQueues; ++q) qa.base[q] = base[q];
+    for (int q = 0; q < nq; ++q) {
+        const long long own = q < tickets ? (tickets - q + nq - 1) / nq : 0;
+        const long long wk = q < workers ? (workers - q + nq - 1) / nq : 0;
+        base[q] += (unsigned long long)(own + wk);   // each worker takes its tickets plus one miss
+    }
 }
 
 int max_coresident_blocks(const void *kernel, int block, size_t lds_bytes) {

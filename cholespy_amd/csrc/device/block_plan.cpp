@@ -1,0 +1,199 @@
+#include "block_plan.hpp"
+
+#include <algorithm>
+#include <numeric>
+#include <stdexcept>
+
+namespace cholespy_amd {
+
+namespace {
+
+// Packed offsets of element (rho, c) of M_B (rows = m + R) in both layouts.
+inline int64_t fwd_off(int64_t rows, int64_t c, int64_t rho) {   // column-major, rows c..rows-1
+    return c * rows - c * (c - 1) / 2 + (rho - c);
+}
+inline int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {      // row-major, cols 0..min(rho,m-1)
+    return rho < m ? rho * (rho + 1) / 2 + c : m * (m + 1) / 2 + (rho - m) * m + c;
+}
+
+}  // namespace
+
+BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const double *val,
+                           const BlockPlanOptions &opt) {
+    BlockPlan P;
+    P.n = n;
+    P.nnz_l = colptr[n];
+    std::vector<int> parent(n, -1), size(n, 1);
+    for (int j = 0; j < n; ++j)
+        if (colptr[j + 1] - colptr[j] > 1) parent[j] = rowidx[colptr[j] + 1];
+    for (int j = 0; j < n; ++j)
+        if (parent[j] >= 0) size[parent[j]] += size[j];
+
+    // 1. Partition into contiguous blocks: small whole subtrees, then chains
+    //    (j and j + 1 share a block when parent(j) = j + 1), capped at max_block.
+    std::vector<int> &start = P.start;
+    int j = 0;
+    while (j < n) {
+        // Is j the first node of a small subtree whose root r has size <= leaf_max
+        // and whose parent is not small? Subtree of r spans [r - size[r] + 1, r].
+        int r = j;
+        while (parent[r] >= 0 && size[parent[r]] <= opt.leaf_max &&
+               parent[r] - size[parent[r]] + 1 == j)
+            r = parent[r];
+        if (size[r] <= opt.leaf_max && r - size[r] + 1 == j && size[r] > 1) {
+            start.push_back(j);
+            j = r + 1;
+            continue;
+        }
+        start.push_back(j);
+        int e = j;
+        while (e + 1 < n && parent[e] == e + 1 && e + 1 - j + 1 <= opt.max_block) ++e;
+        j = e + 1;
+    }
+    const int nb = (int)start.size();
+    start.push_back(n);
+    P.nblocks = nb;
+    std::vector<int> block_of(n);
+    for (int b = 0; b < nb; ++b)
+        for (int i = start[b]; i < start[b + 1]; ++i) block_of[i] = b;
+
+    // 2. Per block: R rows, dense L_BB and L_RB, inverse and products.
+    P.rptr.assign((size_t)nb + 1, 0);
+    P.moff.assign((size_t)nb + 1, 0);
+    std::vector<int> mark(n, -1), pos(n, -1);
+    std::vector<int> rows_b;
+    for (int b = 0; b < nb; ++b) {
+        const int a = start[b], e = start[b + 1];
+        rows_b.clear();
+        for (int c = a; c < e; ++c)
+            for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
+                const int i = rowidx[p];
+                if (i >= e && mark[i] != b) {
+                    mark[i] = b;
+                    rows_b.push_back(i);
+                }
+            }
+        std::sort(rows_b.begin(), rows_b.end());
+        P.rlist.insert(P.rlist.end(), rows_b.begin(), rows_b.end());
+        P.rptr[b + 1] = (int)P.rlist.size();
+        const int64_t m = e - a, R = (int64_t)rows_b.size();
+        P.moff[b + 1] = P.moff[b] + m * (m + 1) / 2 + R * m;
+        P.max_m = std::max<int>(P.max_m, (int)m);
+        P.max_rows = std::max<int>(P.max_rows, (int)(m + R));
+    }
+    P.mfwd.assign((size_t)P.moff[nb], 0.0);
+    P.mbwd.assign((size_t)P.moff[nb], 0.0);
+    {
+        std::vector<double> Lbb, Pb, Linv, Q;
+        for (int b = 0; b < nb; ++b) {
+            const int a = start[b], e = start[b + 1];
+            const int m = e - a;
+            const int R = P.rptr[b + 1] - P.rptr[b];
+            const int *rl = P.rlist.data() + P.rptr[b];
+            for (int q = 0; q < R; ++q) pos[rl[q]] = q;
+            Lbb.assign((size_t)m * m, 0.0);   // row-major
+            Pb.assign((size_t)R * m, 0.0);    // row-major
+            for (int c = a; c < e; ++c)
+                for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
+                    const int i = rowidx[p];
+                    if (i < e) Lbb[(size_t)(i - a) * m + (c - a)] = val[p];
+                    else Pb[(size_t)pos[i] * m + (c - a)] = val[p];
+                }
+            // Linv = Lbb^{-1} by forward substitution on the identity, column by column.
+            Linv.assign((size_t)m * m, 0.0);
+            for (int col = 0; col < m; ++col) {
+                Linv[(size_t)col * m + col] = 1.0 / Lbb[(size_t)col * m + col];
+                for (int i = col + 1; i < m; ++i) {
+                    double s = 0.0;
+                    const double *li = &Lbb[(size_t)i * m];
+                    for (int kk = col; kk < i; ++kk) s += li[kk] * Linv[(size_t)kk * m + col];
+                    Linv[(size_t)i * m + col] = -s / li[i];
+                }
+            }
+            // Q = Pb * Linv (R x m); M_B = [Linv ; -Q].
+            Q.assign((size_t)R * m, 0.0);
+            for (int q = 0; q < R; ++q) {
+                const double *pr = &Pb[(size_t)q * m];
+                double *qr = &Q[(size_t)q * m];
+                for (int kk = 0; kk < m; ++kk) {
+                    const double pk = pr[kk];
+                    if (pk == 0.0) continue;
+                    const double *lr = &Linv[(size_t)kk * m];
+                    for (int c = 0; c <= kk; ++c) qr[c] += pk * lr[c];
+                }
+            }
+            const int64_t rows = (int64_t)m + R;
+            double *F = P.mfwd.data() + P.moff[b];
+            double *Bk = P.mbwd.data() + P.moff[b];
+            for (int rho = 0; rho < rows; ++rho) {
+                const int cmax = rho < m ? rho : m - 1;
+                for (int c = 0; c <= cmax; ++c) {
+                    const double v = rho < m ? Linv[(size_t)rho * m + c] : -Q[(size_t)(rho - m) * m + c];
+                    F[fwd_off(rows, c, rho)] = v;
+                    Bk[bwd_off(m, rho, c)] = v;
+                }
+            }
+            for (int q = 0; q < R; ++q) pos[rl[q]] = -1;
+        }
+    }
+
+    // 3. Contribution lists: slot s (row rlist[s]) feeds row rlist[s].
+    const int nslots = P.rptr[nb];
+    P.contrib_ptr.assign((size_t)n + 1, 0);
+    for (int s = 0; s < nslots; ++s) P.contrib_ptr[(size_t)P.rlist[s] + 1]++;
+    for (int i = 0; i < n; ++i) P.contrib_ptr[i + 1] += P.contrib_ptr[i];
+    P.contrib_slot.resize((size_t)nslots);
+    {
+        std::vector<int> next(P.contrib_ptr.begin(), P.contrib_ptr.end() - 1);
+        for (int s = 0; s < nslots; ++s) P.contrib_slot[next[P.rlist[s]]++] = s;
+    }
+    std::vector<int> slot_block(nslots);
+    for (int b = 0; b < nb; ++b)
+        for (int s = P.rptr[b]; s < P.rptr[b + 1]; ++s) slot_block[s] = b;
+
+    // 4. Block levels: forward depends on blocks with slots for our rows,
+    //    backward on the blocks holding our R rows.
+    P.level_fwd.assign(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+        int lv = 0;
+        for (int i = start[b]; i < start[b + 1]; ++i)
+            for (int p = P.contrib_ptr[i]; p < P.contrib_ptr[i + 1]; ++p)
+                lv = std::max(lv, P.level_fwd[slot_block[P.contrib_slot[p]]] + 1);
+        P.level_fwd[b] = lv;
+    }
+    P.level_bwd.assign(nb, 0);
+    for (int b = nb - 1; b >= 0; --b) {
+        int lv = 0;
+        for (int s = P.rptr[b]; s < P.rptr[b + 1]; ++s)
+            lv = std::max(lv, P.level_bwd[block_of[P.rlist[s]]] + 1);
+        P.level_bwd[b] = lv;
+    }
+
+    // 5. Tasks in level order (stable in block index).
+    std::vector<int> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
+    for (int b : order) {
+        const int rows = (start[b + 1] - start[b]) + (P.rptr[b + 1] - P.rptr[b]);
+        for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows) {
+            P.fwd_task.push_back(b);
+            P.fwd_task.push_back(r0);
+            P.fwd_task.push_back(std::min(rows, r0 + opt.fwd_rows));
+        }
+    }
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
+    for (int b : order) {
+        const int m = start[b + 1] - start[b];
+        for (int c0 = 0; c0 < m; c0 += opt.bwd_cols) {
+            P.bwd_task.push_back(b);
+            P.bwd_task.push_back(c0);
+            P.bwd_task.push_back(std::min(m, c0 + opt.bwd_cols));
+        }
+    }
+    return P;
+}
+
+}  // namespace cholespy_amd

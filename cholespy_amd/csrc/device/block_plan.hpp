@@ -1,0 +1,57 @@
+// Block plan for the supernodal partitioned-inverse solve (host side).
+//
+// The postordered elimination tree of L is cut into contiguous index ranges
+// ("blocks") B = [a, a + m). For each block the plan stores the dense matrix
+//     M_B = [ L_BB^{-1} ; -L_RB L_BB^{-1} ]          ((m + |R_B|) x m)
+// where R_B is the sorted set of rows below the block touched by its columns.
+// With it both triangular passes become one dense product per block:
+//   forward   r_B = (P b)_B + sum of the partials other blocks wrote for B's rows
+//             [y_B ; part_B] = M_B r_B      (part_B: partials for the rows R_B)
+//   backward  z_B = M_B' [y_B ; z_{R_B}]
+// This is the partitioned-inverse idea (Alvarado & Schreiber 1993; Pothen &
+// Alvarado 1992) applied to one dense diagonal block at a time: inside a
+// block there is no sequential dependency left, so the only hand-offs between
+// workgroups are between blocks, whose tree is far shallower than the
+// elimination tree.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace cholespy_amd {
+
+struct BlockPlanOptions {
+    int max_block = 256;    // rows per block (dense inverse is m x m)
+    int leaf_max = 16;      // whole subtrees up to this size become one block
+    int fwd_rows = 64;      // rows of M_B per forward task
+    int bwd_cols = 64;      // columns of M_B per backward task
+};
+
+struct BlockPlan {
+    int n = 0;
+    int nblocks = 0;
+    // Per block b: rows [start[b], start[b+1]); R rows at rlist[rptr[b]..rptr[b+1]).
+    std::vector<int> start, rptr, rlist;
+    // Partial-sum slots: block b owns slots [rptr[b], rptr[b+1]) (one per R row).
+    // contrib_ptr / contrib_slot: for each row i, the slots whose partial sums
+    // belong to row i (CSR over rows).
+    std::vector<int> contrib_ptr, contrib_slot;
+    // Values of M_B in two layouts (one per pass), offsets per block.
+    //   forward : column-major packed  (column c holds rows c .. m+|R|-1)
+    //   backward: row-major packed     (row rho holds columns 0 .. min(rho, m-1))
+    std::vector<int64_t> moff;            // same size for both layouts
+    std::vector<double> mfwd, mbwd;
+    // Block level in the block tree: forward level (height) and backward level (depth).
+    std::vector<int> level_fwd, level_bwd;
+    // Tasks in schedule order: (block, first, last) with rows of M_B
+    // (forward) or columns of M_B (backward) in [first, last).
+    std::vector<int> fwd_task, bwd_task;   // 3 ints per task
+    int max_m = 0, max_rows = 0;           // largest m and m + |R|
+    int64_t nnz_l = 0;
+};
+
+// Builds the plan from L in compressed-column form (rows ascending,
+// diagonal first), postordered so that parent(j) > j.
+BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const double *val,
+                           const BlockPlanOptions &opt);
+
+}  // namespace cholespy_amd

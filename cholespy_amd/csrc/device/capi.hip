@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../../include/cholespy_amd.h"
+#include "block_plan.hpp"
 #include "common.hpp"
 #include "launchers.hpp"
 
@@ -51,6 +52,39 @@ void dev_free(T *&p) {
     p = nullptr;
 }
 
+template <typename T>
+hipError_t upload(T **dst, const T *src, size_t count) {
+    hipError_t e = dev_alloc(dst, count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice);
+}
+
+// Device copy of a BlockPlan (block_plan.hpp) in the solver's precision.
+struct DeviceBlockPlan {
+    int nblocks = 0, nslots = 0, max_rows = 0;
+    int n_fwd_tasks = 0, n_bwd_tasks = 0;
+    int levels_fwd = 0, levels_bwd = 0;
+    int64_t m_entries = 0;
+    int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
+    int64_t *moff = nullptr;
+    void *mfwd = nullptr, *mbwd = nullptr;
+    int *cptr = nullptr, *cslot = nullptr;
+    int *ftask = nullptr, *btask = nullptr;
+
+    void release() {
+        dev_free(start);
+        dev_free(rptr);
+        dev_free(rlist);
+        dev_free(moff);
+        dev_free(mfwd);
+        dev_free(mbwd);
+        dev_free(cptr);
+        dev_free(cslot);
+        dev_free(ftask);
+        dev_free(btask);
+    }
+};
+
 }  // namespace
 
 struct cholespy_gpu_solver {
@@ -72,8 +106,25 @@ struct cholespy_gpu_solver {
     int *lvl1 = nullptr;                  // analysis scratch: level + 1
     int *status = nullptr;                // [0] error code, [1] abort flag
     void *y = nullptr, *z = nullptr;      // per-solve work buffers (n x k)
+    void *part = nullptr;                 // block path partial slots (nslots x k)
     int k_cur = 0;                        // layout the work buffers hold
+    int path = CHOLESPY_PATH_BLOCKS;
+    int buffers_path = -1;                // path whose invariants the buffers hold
     size_t work_elems = 0;                // capacity of y and z in elements
+    size_t part_elems = 0;
+    DeviceBlockPlan plan;
+    // Full row-compressed A in FP64 (residual of the refinement path).
+    int *arowptr = nullptr, *acols = nullptr;
+    double *avals = nullptr;
+    bool has_matrix = false;
+    float *r32 = nullptr;                 // FP32 right-hand side of correction solves
+    size_t r32_elems = 0;
+    int grid_rows[2][2] = {{0, 0}, {0, 0}};   // [lower][k > 64]
+    int grid_blocks[2] = {0, 0};              // [forward]
+    // Ticket queues per kernel: [0..1] analysis upper/lower, [2..3] row pass
+    // backward/forward, [4..5] block pass backward/forward.
+    unsigned long long *qctr = nullptr;
+    QueueState queues[6];
     std::vector<int> host_levels[2], host_row_level[2];
 };
 
@@ -97,9 +148,6 @@ int cholespy_init(void) {
     HIP_TRY(hipGetDevice(&g_device));
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, g_device));
-    int coop = 0;
-    HIP_TRY(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, g_device));
-    if (!coop) return fail(CHOLESPY_ERR_NO_DEVICE, "device does not support cooperative launch");
     g_initialised = true;
     return CHOLESPY_OK;
 }
@@ -133,6 +181,11 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
     if (e == hipSuccess) e = hipMemset(s->status, 0, 2 * sizeof(int));
     if (e == hipSuccess) e = dev_alloc(&s->perm, (size_t)n);
     if (e == hipSuccess) e = dev_alloc(&s->lvl1, (size_t)n);
+    const size_t qwords = 6 * (size_t)kMaxQueues * kQueueStride;
+    if (e == hipSuccess) e = dev_alloc(&s->qctr, qwords);
+    if (e == hipSuccess) e = hipMemset(s->qctr, 0, qwords * sizeof(unsigned long long));
+    if (e == hipSuccess)
+        for (int i = 0; i < 6; ++i) s->queues[i].ctr = s->qctr + (size_t)i * kMaxQueues * kQueueStride;
     if (e != hipSuccess) {
         cholespy_solver_destroy(s);
         return hip_fail(e, "cholespy_solver_create");
@@ -147,6 +200,7 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->status);
     dev_free(s->perm);
     dev_free(s->lvl1);
+    dev_free(s->qctr);
     dev_free(s->lrowptr);
     dev_free(s->lcols);
     dev_free(s->lvals);
@@ -157,6 +211,12 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->order[1]);
     dev_free(s->y);
     dev_free(s->z);
+    dev_free(s->part);
+    dev_free(s->arowptr);
+    dev_free(s->acols);
+    dev_free(s->avals);
+    dev_free(s->r32);
+    s->plan.release();
     delete s;
     return CHOLESPY_OK;
 }
@@ -233,6 +293,43 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     s->nnz = nnz;
     s->uploaded = true;
     s->analysed[0] = s->analysed[1] = false;
+
+    // Block plan for the partitioned-inverse path.
+    BlockPlanOptions opt;
+    BlockPlan P = build_block_plan(n, colptr, rowidx, val, opt);
+    DeviceBlockPlan &d = s->plan;
+    d.release();
+    d.nblocks = P.nblocks;
+    d.nslots = P.rptr[P.nblocks];
+    d.max_rows = P.max_rows;
+    d.n_fwd_tasks = (int)(P.fwd_task.size() / 3);
+    d.n_bwd_tasks = (int)(P.bwd_task.size() / 3);
+    d.levels_fwd = d.levels_bwd = 0;
+    for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
+    for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
+    d.m_entries = P.moff[P.nblocks];
+    HIP_TRY(upload(&d.start, P.start.data(), P.start.size()));
+    HIP_TRY(upload(&d.rptr, P.rptr.data(), P.rptr.size()));
+    HIP_TRY(upload(&d.rlist, P.rlist.data(), P.rlist.size()));
+    HIP_TRY(upload(&d.moff, P.moff.data(), P.moff.size()));
+    HIP_TRY(upload(&d.cptr, P.contrib_ptr.data(), P.contrib_ptr.size()));
+    HIP_TRY(upload(&d.cslot, P.contrib_slot.data(), P.contrib_slot.size()));
+    HIP_TRY(upload(&d.ftask, P.fwd_task.data(), P.fwd_task.size()));
+    HIP_TRY(upload(&d.btask, P.bwd_task.data(), P.bwd_task.size()));
+    const size_t me = (size_t)d.m_entries;
+    if (s->fbytes == 4) {
+        std::vector<float> f(me), b(me);
+        for (size_t i = 0; i < me; ++i) {
+            f[i] = (float)P.mfwd[i];
+            b[i] = (float)P.mbwd[i];
+        }
+        HIP_TRY(upload(reinterpret_cast<float **>(&d.mfwd), f.data(), me));
+        HIP_TRY(upload(reinterpret_cast<float **>(&d.mbwd), b.data(), me));
+    } else {
+        HIP_TRY(upload(reinterpret_cast<double **>(&d.mfwd), P.mfwd.data(), me));
+        HIP_TRY(upload(reinterpret_cast<double **>(&d.mbwd), P.mbwd.data(), me));
+    }
+    s->buffers_path = -1;
     return CHOLESPY_OK;
 }
 
@@ -253,8 +350,9 @@ int cholespy_solver_analyze(cholespy_gpu_solver *s, int lower, int *n_levels) {
     HIP_TRY(hipMemset(s->status, 0, 2 * sizeof(int)));
     const int cap = max_coresident_blocks(level_analysis_kernel(lower != 0), kRowBlock, 0);
     if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for level analysis");
-    const int grid = std::max(1, std::min(cap, (n + 3) / 4));
-    HIP_TRY(launch_level_analysis(lower != 0, n, rowptr, cols, s->lvl1, s->status, grid, nullptr));
+    const int grid = std::max(1, std::min(cap, (n + 4 * kRowChunk - 1) / (4 * kRowChunk)));
+    HIP_TRY(launch_level_analysis(lower != 0, n, rowptr, cols, s->lvl1, s->status, grid,
+                                  s->queues[li], nullptr));
     std::vector<int> lvl1((size_t)n);
     int status[2] = {0, 0};
     HIP_TRY(hipMemcpy(lvl1.data(), s->lvl1, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
@@ -303,10 +401,15 @@ int cholespy_solver_check(cholespy_gpu_solver *s) {
     HIP_TRY(hipMemcpy(status, s->status, sizeof(status), hipMemcpyDeviceToHost));
     if (status[0] != 0 || status[1] != 0) {
         HIP_TRY(hipMemset(s->status, 0, 2 * sizeof(int)));
+        HIP_TRY(hipMemset(s->qctr, 0, 6 * (size_t)kMaxQueues * kQueueStride * sizeof(unsigned long long)));
+        for (auto &q : s->queues)
+            for (auto &b : q.base) b = 0;
         // Work buffers may hold partial results: restore the sentinel state.
         if (s->work_elems) {
             HIP_TRY(launch_fill_sentinel(s->y, s->work_elems, s->fbytes, nullptr));
             HIP_TRY(launch_fill_sentinel(s->z, s->work_elems, s->fbytes, nullptr));
+            if (s->part && s->part_elems)
+                HIP_TRY(launch_fill_sentinel(s->part, s->part_elems, s->fbytes, nullptr));
             HIP_TRY(hipDeviceSynchronize());
         }
         return fail(CHOLESPY_ERR_TIMEOUT, "a solve kernel exceeded its bounded wait");
@@ -318,33 +421,39 @@ int cholespy_solver_check(cholespy_gpu_solver *s) {
 
 namespace {
 
-int solve_impl(cholespy_gpu_solver *s, int fbytes, int k, const void *b, void *x, void *stream) {
-    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
-    if (s->fbytes != fbytes)
-        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "solver precision does not match the call");
-    if (k > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
-    if (k < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative number of RHS");
-    if (!s->analysed[0] || !s->analysed[1])
-        return fail(CHOLESPY_ERR_STATE, "solve called before both passes were analysed");
-    const int n = s->n;
-    if (k == 0 || n == 0) return CHOLESPY_OK;
-    if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    const size_t elems = (size_t)n * (size_t)k;
+// Makes the work buffers hold n x k elements in the layout and sentinel
+// state the current path expects.
+int prepare_buffers(cholespy_gpu_solver *s, int k, hipStream_t st) {
+    const int fbytes = s->fbytes;
+    const size_t elems = (size_t)s->n * (size_t)k;
     if (elems > s->work_elems) {
-        dev_free(reinterpret_cast<char *&>(s->y));
-        dev_free(reinterpret_cast<char *&>(s->z));
+        dev_free(s->y);
+        dev_free(s->z);
         HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), elems * (size_t)fbytes));
         HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), elems * (size_t)fbytes));
         s->work_elems = elems;
         s->k_cur = 0;
     }
-    if (k != s->k_cur) {
-        // The buffers are indexed row * k + col: refill them for the new layout.
+    const size_t pe = (size_t)s->plan.nslots * (size_t)k;
+    if (s->path == CHOLESPY_PATH_BLOCKS && pe > s->part_elems) {
+        dev_free(s->part);
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->part), std::max<size_t>(pe, 1) * (size_t)fbytes));
+        s->part_elems = pe;
+        s->k_cur = 0;
+    }
+    if (k != s->k_cur || s->buffers_path != s->path) {
+        // Buffers are indexed row * k + col: refill them for the new layout.
         HIP_TRY(launch_fill_sentinel(s->y, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->z, elems, fbytes, st));
+        if (s->part && pe) HIP_TRY(launch_fill_sentinel(s->part, pe, fbytes, st));
         s->k_cur = k;
+        s->buffers_path = s->path;
     }
+    return CHOLESPY_OK;
+}
+
+int solve_rows(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStream_t st, int only_pass = -1) {
+    const int fbytes = s->fbytes, n = s->n;
     RowPassBuffers bf;
     bf.n = n;
     bf.k = k;
@@ -355,22 +464,190 @@ int solve_impl(cholespy_gpu_solver *s, int fbytes, int k, const void *b, void *x
     bf.z = s->z;
     bf.status = s->status;
     for (int pass = 0; pass < 2; ++pass) {
+        if (only_pass >= 0 && pass != only_pass) continue;
         const bool lower = pass == 0;
         bf.order = s->order[lower ? 1 : 0];
         bf.rowptr = lower ? s->lrowptr : s->urowptr;
         bf.cols = lower ? s->lcols : s->ucols;
         bf.vals = lower ? s->lvals : s->uvals;
-        const int cap = max_coresident_blocks(row_pass_kernel(fbytes, lower, k), kRowBlock, 0);
-        if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for the solve");
-        const int grid = std::max(1, std::min(cap, (n + 3) / 4));
-        HIP_TRY(launch_row_pass(fbytes, lower, bf, grid, st));
+        int &grid = s->grid_rows[lower ? 1 : 0][k > kWave ? 1 : 0];
+        if (grid == 0) {
+            const int cap = max_coresident_blocks(row_pass_kernel(fbytes, lower, k), kRowBlock, 0);
+            if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for the solve");
+            grid = std::max(1, std::min(cap, (n + 4 * kRowChunk - 1) / (4 * kRowChunk)));
+        }
+        HIP_TRY(launch_row_pass(fbytes, lower, bf, grid, s->queues[2 + (lower ? 1 : 0)], st));
     }
     return CHOLESPY_OK;
+}
+
+int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStream_t st, int xmode = 0,
+                 int only_pass = -1) {
+    const int fbytes = s->fbytes;
+    const DeviceBlockPlan &d = s->plan;
+    BlockPassBuffers bf;
+    bf.k = k;
+    bf.max_rows = d.max_rows;
+    bf.start = d.start;
+    bf.rptr = d.rptr;
+    bf.rlist = d.rlist;
+    bf.moff = d.moff;
+    bf.cptr = d.cptr;
+    bf.cslot = d.cslot;
+    bf.perm = s->perm;
+    bf.b = b;
+    bf.x = x;
+    bf.y = s->y;
+    bf.z = s->z;
+    bf.part = s->part;
+    bf.status = s->status;
+    bf.xmode = xmode;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (only_pass >= 0 && pass != only_pass) continue;
+        const bool forward = pass == 0;
+        bf.ntasks = forward ? d.n_fwd_tasks : d.n_bwd_tasks;
+        bf.task = forward ? d.ftask : d.btask;
+        bf.M = forward ? d.mfwd : d.mbwd;
+        int &grid = s->grid_blocks[forward ? 1 : 0];
+        if (grid == 0) {
+            const size_t lds = block_pass_lds(fbytes, d.max_rows);
+            const int cap = max_coresident_blocks(block_pass_kernel(fbytes, forward), 256, lds);
+            if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for the block solve");
+            grid = std::max(1, std::min(cap, bf.ntasks));
+        }
+        HIP_TRY(launch_block_pass(fbytes, forward, bf, grid, s->queues[4 + (forward ? 1 : 0)], st));
+    }
+    return CHOLESPY_OK;
+}
+
+int solve_impl(cholespy_gpu_solver *s, int fbytes, int k, const void *b, void *x, void *stream,
+               int only_pass = -1) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    if (s->fbytes != fbytes)
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "solver precision does not match the call");
+    if (k > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
+    if (k < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative number of RHS");
+    if (!s->analysed[0] || !s->analysed[1])
+        return fail(CHOLESPY_ERR_STATE, "solve called before both passes were analysed");
+    if (k == 0 || s->n == 0) return CHOLESPY_OK;
+    if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int rc = prepare_buffers(s, k, st);
+    if (rc != CHOLESPY_OK) return rc;
+    return s->path == CHOLESPY_PATH_ROWS ? solve_rows(s, k, b, x, st, only_pass)
+                                         : solve_blocks(s, k, b, x, st, 0, only_pass);
 }
 
 }  // namespace
 
 extern "C" {
+
+int cholespy_solver_upload_matrix(cholespy_gpu_solver *s, const int *colptr, const int *rowidx,
+                                  const double *val, int64_t nnz) {
+    if (!s || !colptr || (!rowidx && nnz) || (!val && nnz))
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    const int n = s->n;
+    if (colptr[0] != 0 || (int64_t)colptr[n] != nnz)
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "colptr does not match nnz");
+    for (int j = 0; j < n; ++j) {
+        if (colptr[j + 1] < colptr[j]) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "colptr not monotone");
+        for (int p = colptr[j]; p < colptr[j + 1]; ++p)
+            if (rowidx[p] < j || rowidx[p] >= n)
+                return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "matrix entry outside the lower triangle");
+    }
+    // Full symmetric rows: (i, j) and (j, i) for every stored i > j.
+    std::vector<int> rowptr((size_t)n + 1, 0);
+    for (int j = 0; j < n; ++j)
+        for (int p = colptr[j]; p < colptr[j + 1]; ++p) {
+            rowptr[(size_t)rowidx[p] + 1]++;
+            if (rowidx[p] != j) rowptr[(size_t)j + 1]++;
+        }
+    for (int i = 0; i < n; ++i) rowptr[i + 1] += rowptr[i];
+    std::vector<int> cols((size_t)rowptr[n]);
+    std::vector<double> vals((size_t)rowptr[n]);
+    {
+        std::vector<int> next(rowptr.begin(), rowptr.end() - 1);
+        // Column j's entries go to rows i (column index j, ascending j) and,
+        // mirrored, to row j (column indices i > j, ascending): each row
+        // ends up sorted by column.
+        for (int j = 0; j < n; ++j)
+            for (int p = colptr[j]; p < colptr[j + 1]; ++p) {
+                const int i = rowidx[p];
+                cols[next[i]] = j;
+                vals[next[i]++] = val[p];
+            }
+        for (int j = 0; j < n; ++j)
+            for (int p = colptr[j]; p < colptr[j + 1]; ++p) {
+                const int i = rowidx[p];
+                if (i == j) continue;
+                cols[next[j]] = i;
+                vals[next[j]++] = val[p];
+            }
+    }
+    dev_free(s->arowptr);
+    dev_free(s->acols);
+    dev_free(s->avals);
+    HIP_TRY(upload(&s->arowptr, rowptr.data(), rowptr.size()));
+    HIP_TRY(upload(&s->acols, cols.data(), cols.size()));
+    HIP_TRY(upload(&s->avals, vals.data(), vals.size()));
+    s->has_matrix = true;
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_solve_refined(cholespy_gpu_solver *s, int k, const double *b, double *x,
+                                  int iterations, void *stream) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    if (s->fbytes != 4) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "refined solves need a float32 solver");
+    if (!s->has_matrix) return fail(CHOLESPY_ERR_STATE, "refined solve called before upload_matrix");
+    if (k > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
+    if (k < 0 || iterations < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative argument");
+    if (k == 0 || s->n == 0) return CHOLESPY_OK;
+    if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
+    if (s->path != CHOLESPY_PATH_BLOCKS) return fail(CHOLESPY_ERR_STATE, "refined solves use the block path");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    int rc = prepare_buffers(s, k, st);
+    if (rc != CHOLESPY_OK) return rc;
+    const size_t elems = (size_t)s->n * (size_t)k;
+    if (elems > s->r32_elems) {
+        dev_free(s->r32);
+        HIP_TRY(dev_alloc(&s->r32, elems));
+        s->r32_elems = elems;
+    }
+    // x = A^{-1} b in FP32, then `iterations` corrections x += A^{-1}(b - A x).
+    HIP_TRY(launch_cast_to_f32(b, s->r32, elems, st));
+    rc = solve_blocks(s, k, s->r32, x, st, 1);
+    if (rc != CHOLESPY_OK) return rc;
+    for (int it = 0; it < iterations; ++it) {
+        HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, st));
+        rc = solve_blocks(s, k, s->r32, x, st, 2);
+        if (rc != CHOLESPY_OK) return rc;
+    }
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_solve_pass(cholespy_gpu_solver *s, int pass, int n_rhs, const void *b, void *x,
+                               void *stream) {
+    if (pass != 0 && pass != 1) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "pass must be 0 or 1");
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    return solve_impl(s, s->fbytes, n_rhs, b ? b : x, x ? x : const_cast<void *>(b), stream, pass);
+}
+
+int cholespy_solver_set_path(cholespy_gpu_solver *s, int path) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    if (path != CHOLESPY_PATH_ROWS && path != CHOLESPY_PATH_BLOCKS)
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "unknown solve path");
+    s->path = path;
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_plan_info(const cholespy_gpu_solver *s, int64_t *info, int count) {
+    if (!s || !info) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    const DeviceBlockPlan &d = s->plan;
+    const int64_t vals[] = {d.nblocks, d.m_entries, d.levels_fwd, d.levels_bwd,
+                            d.n_fwd_tasks, d.n_bwd_tasks, d.nslots, d.max_rows};
+    for (int i = 0; i < count && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
+    return CHOLESPY_OK;
+}
 
 int cholespy_solver_solve_f32(cholespy_gpu_solver *s, int n_rhs, const float *b, float *x, void *stream) {
     return solve_impl(s, 4, n_rhs, b, x, stream);

@@ -119,6 +119,34 @@ __device__ __forceinline__ int wait_nonzero(const int *p, int *status) {
     }
 }
 
+// Dynamic work queues. Tickets (rows, row chunks or block tasks, numbered in
+// dependency order) are taken in increasing order, so a ticket is only ever
+// held by a running worker and every wait targets a ticket that is running
+// or finished: the schedule needs no co-residency guarantee (the occupancy
+// reported for cooperative launches was measured to over-state residency on
+// MI355X). To avoid serialising on one atomic address, tickets are spread
+// over nq queues: ticket t belongs to queue t mod nq, worker w serves queue
+// w mod nq (the first nq workers are dispatched first, so every queue has a
+// running worker). Counters only grow: each launch passes the value every
+// counter had at its start (`base`), so no reset is needed between launches.
+constexpr int kMaxQueues = 16;
+constexpr int kQueueStride = 16;   // 128 bytes between counters
+
+struct QueueArgs {
+    unsigned long long *ctr;
+    unsigned long long base[kMaxQueues];
+    int nq;
+};
+
+// Next ticket of queue q, or -1 when the queue is exhausted (n tickets in all).
+__device__ __forceinline__ int take_ticket(const QueueArgs &qa, int q, int n) {
+    const unsigned long long v =
+        __hip_atomic_fetch_add(qa.ctr + (size_t)q * kQueueStride, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long j = (long long)(v - qa.base[q]);
+    const long long t = j * qa.nq + q;
+    return t < n ? (int)t : -1;
+}
+
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));

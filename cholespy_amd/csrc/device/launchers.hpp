@@ -6,8 +6,22 @@
 
 namespace cholespy_amd {
 
+struct QueueArgs;   // common.hpp
+
+// Host-side bookkeeping of one kernel's ticket queues (see common.hpp).
+struct QueueState {
+    unsigned long long *ctr = nullptr;   // device, kMaxQueues * 16 counters
+    unsigned long long base[16] = {0};
+    int nq = 16;
+    // Fills the launch arguments and advances the bases past this launch, in
+    // which `workers` workers take `tickets` tickets in all.
+    void next_launch(QueueArgs &qa, int workers, int tickets);
+};
+
 // Threads per workgroup of the persistent row kernels (4 wavefronts).
 constexpr int kRowBlock = 256;
+// Rows per ticket of the row kernels (consecutive in dependency order).
+constexpr int kRowChunk = 4;
 
 // Largest co-resident grid (workgroups) for a persistent kernel.
 int max_coresident_blocks(const void *kernel, int block, size_t lds_bytes);
@@ -16,7 +30,8 @@ int max_coresident_blocks(const void *kernel, int block, size_t lds_bytes);
 // on exit lvl1[r] = level(r) + 1.
 const void *level_analysis_kernel(bool lower);
 hipError_t launch_level_analysis(bool lower, int n, const int *rowptr, const int *cols,
-                                 int *lvl1, int *status, int grid, hipStream_t stream);
+                                 int *lvl1, int *status, int grid, QueueState &queues,
+                                 hipStream_t stream);
 
 // Row-per-wavefront level-scheduled triangular pass (kernels/kernels.cu:64-202).
 struct RowPassBuffers {
@@ -34,7 +49,34 @@ struct RowPassBuffers {
 };
 const void *row_pass_kernel(int float_bytes, bool lower, int k);
 hipError_t launch_row_pass(int float_bytes, bool lower, const RowPassBuffers &buf, int grid,
-                           hipStream_t stream);
+                           QueueState &queues, hipStream_t stream);
+
+// Partitioned-inverse block passes (block_plan.hpp / trsv_blocks.hip).
+constexpr int kBlockKT = 4;   // right-hand-side columns per LDS tile
+struct BlockPassBuffers {
+    int ntasks = 0, k = 0, max_rows = 0;
+    const int *task = nullptr;     // (block, first, last) per task, in ticket order
+    const int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
+    const int64_t *moff = nullptr;
+    const void *M = nullptr;       // forward or backward layout of M_B
+    const int *cptr = nullptr, *cslot = nullptr;
+    const int *perm = nullptr;
+    const void *b = nullptr;
+    void *y = nullptr, *z = nullptr, *part = nullptr, *x = nullptr;
+    int *status = nullptr;
+    // Backward epilogue: 0 = x (solver precision) = z; 1 = x (double) = z;
+    // 2 = x (double) += z (refinement update).
+    int xmode = 0;
+};
+size_t block_pass_lds(int float_bytes, int max_rows);
+const void *block_pass_kernel(int float_bytes, bool forward);
+hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffers &buf, int grid,
+                             QueueState &queues, hipStream_t stream);
+
+// Mixed-precision refinement helpers (refine.hip).
+hipError_t launch_cast_to_f32(const double *in, float *out, size_t count, hipStream_t stream);
+hipError_t launch_residual_f64(int n, int k, const int *rowptr, const int *cols, const double *vals,
+                               const double *b, const double *x, float *r32, hipStream_t stream);
 
 // Fills n words of `bytes` (4 or 8) with the sentinel of that width.
 hipError_t launch_fill_sentinel(void *p, size_t n, int bytes, hipStream_t stream);

@@ -1,0 +1,320 @@
+// Partitioned-inverse block passes on gfx950 (the plan is in block_plan.hpp).
+//
+// Forward task (block B, rows [first, last) of M_B):
+//   r_B[i] = b[perm[a+i]] + sum of the published partials for row a+i
+//   out    = M_B[first:last, :] r_B
+//   rows < m  -> y[a+rho]                   (read by the backward pass only)
+//   rows >= m -> partial slot for row R_B[rho-m]   (published, waited on)
+// Backward task (block B, columns [first, last) of M_B):
+//   z_B[c] = sum_rho M_B[rho, c] * v[rho],  v = [y_B ; z_{R_B}]
+//   z published (waited on by descendants), x[perm[a+c]] = z_B[c].
+// No task waits on a task of its own block, and every wait targets a block of
+// a strictly lower block level, i.e. an earlier ticket; workgroups take
+// tickets in order from the queues of common.hpp (the next one prefetched
+// while the current task runs), so every awaited task is running or done and
+// the schedule cannot deadlock.
+//
+// Each task is one workgroup of 256 threads: 64 rows (forward) or 64 columns
+// (backward) x 4 segments of the reduction dimension, so every M load is a
+// 256-byte coalesced wavefront access, and the 4 segment sums are combined in
+// LDS. Right-hand sides are processed in tiles of KT columns held in LDS.
+//
+// Sentinel hand-off as in common.hpp. The forward pass resets z for its rows,
+// the backward pass resets the partial slots of its block, so the next solve
+// starts from the all-sentinel state without a memset.
+#include "common.hpp"
+#include "launchers.hpp"
+
+namespace cholespy_amd {
+
+namespace {
+
+constexpr int kTaskThreads = 256;
+constexpr int kLanes = 64;      // rows / columns per task
+constexpr int kSegs = 4;        // segments of the reduction dimension
+constexpr int kGatherUnroll = 8;
+
+template <typename F>
+struct BlockPassArgs {
+    int ntasks, k, max_rows;
+    const int *task;        // (block, first, last) per task
+    const int *start;       // block row ranges
+    const int *rptr;        // R list / partial slot ranges
+    const int *rlist;
+    const int64_t *moff;
+    const F *M;             // forward: column-major packed; backward: row-major packed
+    const int *cptr;        // contribution lists (forward)
+    const int *cslot;
+    const int *perm;
+    const F *b;
+    F *y, *z, *part;
+    void *x;
+    int *status;
+    int xmode;
+    QueueArgs qa;
+};
+
+// Ticket handling per workgroup: thread 0 takes tickets, the LDS word s_next
+// broadcasts them at task boundaries (after a barrier).
+__device__ __forceinline__ int wg_first_ticket(const QueueArgs &qa, int q, int n, int *s_next) {
+    if (threadIdx.x == 0) *s_next = take_ticket(qa, q, n);
+    __syncthreads();
+    return *s_next;
+}
+
+__device__ __forceinline__ int64_t fwd_off(int64_t rows, int64_t c, int64_t rho) {
+    return c * rows - c * (c - 1) / 2 + (rho - c);
+}
+__device__ __forceinline__ int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {
+    return rho < m ? rho * (rho + 1) / 2 + c : m * (m + 1) / 2 + (rho - m) * m + c;
+}
+
+// Sum of the partials in slots cslot[p0..p1) for column col, in slot order.
+template <typename F>
+__device__ __forceinline__ F gather_partials(const BlockPassArgs<F> &a, int p0, int p1, int col) {
+    using U = typename Bits<F>::U;
+    F s = F(0);
+    for (int p = p0; p < p1; p += kGatherUnroll) {
+        U u[kGatherUnroll];
+        int slot[kGatherUnroll];
+#pragma unroll
+        for (int j = 0; j < kGatherUnroll; ++j) {
+            slot[j] = p + j < p1 ? a.cslot[p + j] : -1;
+            u[j] = slot[j] >= 0 ? load_bits(&a.part[(size_t)slot[j] * a.k + col]) : U(0);
+        }
+#pragma unroll
+        for (int j = 0; j < kGatherUnroll; ++j) {
+            F v = from_bits<F>(u[j]);
+            if (u[j] == Bits<F>::sentinel) v = wait_value(&a.part[(size_t)slot[j] * a.k + col], a.status);
+            s += v;
+        }
+    }
+    return s;
+}
+
+template <typename F, int KT>
+__global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    F *rv = reinterpret_cast<F *>(smem);                   // max_rows x KT (only m used)
+    F *red = rv + (size_t)a.max_rows * KT;                 // 256 x KT
+    __shared__ int s_next;
+    const int tid = threadIdx.x;
+    const int lr = tid & (kLanes - 1), seg = tid / kLanes;
+    const int qid = blockIdx.x % a.qa.nq;
+    int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
+    while (tk >= 0) {
+        int pre = 0;
+        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
+        const int blk = a.task[3 * tk], first = a.task[3 * tk + 1], last = a.task[3 * tk + 2];
+        const int a0 = a.start[blk];
+        const int m = a.start[blk + 1] - a0;
+        const int R = a.rptr[blk + 1] - a.rptr[blk];
+        const int rows = m + R;
+        const F *M = a.M + a.moff[blk];
+        for (int t0 = 0; t0 < a.k; t0 += KT) {
+            const int kt = min(KT, a.k - t0);
+            for (int idx = tid; idx < m * KT; idx += kTaskThreads) {
+                const int i = idx / KT, t = idx % KT;
+                F v = F(0);
+                if (t < kt) {
+                    const int row = a0 + i, col = t0 + t;
+                    v = a.b[(size_t)a.perm[row] * a.k + col] +
+                        gather_partials(a, a.cptr[row], a.cptr[row + 1], col);
+                }
+                rv[idx] = v;
+            }
+            __syncthreads();
+            const int rho = first + lr;
+            F acc[KT];
+#pragma unroll
+            for (int t = 0; t < KT; ++t) acc[t] = F(0);
+            if (rho < last) {
+                const int cend = rho < m ? rho + 1 : m;
+                const int cq = (m + kSegs - 1) / kSegs;
+                const int cb = seg * cq, ce = min(cend, cb + cq);
+                for (int c = cb; c < ce; ++c) {
+                    const F mv = M[fwd_off(rows, c, rho)];
+#pragma unroll
+                    for (int t = 0; t < KT; ++t) acc[t] = fma(mv, rv[c * KT + t], acc[t]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < KT; ++t) red[tid * KT + t] = acc[t];
+            __syncthreads();
+            if (seg == 0 && rho < last) {
+#pragma unroll
+                for (int t = 0; t < KT; ++t) {
+                    if (t >= kt) break;
+                    F s = red[lr * KT + t];
+#pragma unroll
+                    for (int g = 1; g < kSegs; ++g) s += red[(g * kLanes + lr) * KT + t];
+                    const int col = t0 + t;
+                    if (rho < m) {
+                        const size_t o = (size_t)(a0 + rho) * a.k + col;
+                        a.y[o] = s;
+                        a.z[o] = from_bits<F>(Bits<F>::sentinel);
+                    } else {
+                        publish(&a.part[(size_t)(a.rptr[blk] + rho - m) * a.k + col], s);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) s_next = pre;
+        __syncthreads();
+        tk = s_next;
+        __syncthreads();
+    }
+}
+
+template <typename F, int KT>
+__global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> a) {
+    using U = typename Bits<F>::U;
+    extern __shared__ __align__(16) unsigned char smem[];
+    F *vs = reinterpret_cast<F *>(smem);                   // max_rows x KT
+    F *red = vs + (size_t)a.max_rows * KT;                 // 256 x KT
+    __shared__ int s_next;
+    const int tid = threadIdx.x;
+    const int lr = tid & (kLanes - 1), seg = tid / kLanes;
+    const int qid = blockIdx.x % a.qa.nq;
+    int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
+    while (tk >= 0) {
+        int pre = 0;
+        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
+        const int blk = a.task[3 * tk], first = a.task[3 * tk + 1], last = a.task[3 * tk + 2];
+        const int a0 = a.start[blk];
+        const int m = a.start[blk + 1] - a0;
+        const int r0 = a.rptr[blk];
+        const int R = a.rptr[blk + 1] - r0;
+        const int rows = m + R;
+        const F *M = a.M + a.moff[blk];
+        for (int t0 = 0; t0 < a.k; t0 += KT) {
+            const int kt = min(KT, a.k - t0);
+            // v = [y_B ; z_R]: y from the forward pass (complete), z waited on.
+            for (int base = tid; base < rows * KT; base += kTaskThreads * kGatherUnroll) {
+                U u[kGatherUnroll];
+                int addr[kGatherUnroll];
+#pragma unroll
+                for (int j = 0; j < kGatherUnroll; ++j) {
+                    const int idx = base + j * kTaskThreads;
+                    const int rho = idx / KT, t = idx % KT;
+                    addr[j] = -1;
+                    u[j] = U(0);
+                    if (idx < rows * KT && t < kt) {
+                        if (rho < m) {
+                            u[j] = as_bits(a.y[(size_t)(a0 + rho) * a.k + t0 + t]);
+                        } else {
+                            addr[j] = a.rlist[r0 + rho - m] * a.k + t0 + t;
+                            u[j] = load_bits(&a.z[addr[j]]);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int j = 0; j < kGatherUnroll; ++j) {
+                    const int idx = base + j * kTaskThreads;
+                    if (idx >= rows * KT) break;
+                    F v = from_bits<F>(u[j]);
+                    if (addr[j] >= 0 && u[j] == Bits<F>::sentinel) v = wait_value(&a.z[addr[j]], a.status);
+                    vs[idx] = v;
+                }
+            }
+            __syncthreads();
+            const int c = first + lr;
+            F acc[KT];
+#pragma unroll
+            for (int t = 0; t < KT; ++t) acc[t] = F(0);
+            if (c < last) {
+                const int rq = (rows + kSegs - 1) / kSegs;
+                const int rb = max(seg * rq, c), re = min(rows, seg * rq + rq);
+                for (int rho = rb; rho < re; ++rho) {
+                    const F mv = M[bwd_off(m, rho, c)];
+#pragma unroll
+                    for (int t = 0; t < KT; ++t) acc[t] = fma(mv, vs[rho * KT + t], acc[t]);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < KT; ++t) red[tid * KT + t] = acc[t];
+            __syncthreads();
+            if (seg == 0 && c < last) {
+#pragma unroll
+                for (int t = 0; t < KT; ++t) {
+                    if (t >= kt) break;
+                    F s = red[lr * KT + t];
+#pragma unroll
+                    for (int g = 1; g < kSegs; ++g) s += red[(g * kLanes + lr) * KT + t];
+                    const int col = t0 + t;
+                    publish(&a.z[(size_t)(a0 + c) * a.k + col], s);
+                    const size_t o = (size_t)a.perm[a0 + c] * a.k + col;
+                    if (a.xmode == 0) {
+                        static_cast<F *>(a.x)[o] = s;
+                    } else {
+                        double *x64 = static_cast<double *>(a.x);
+                        x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        // Restore the sentinel in this block's partial slots for the next solve.
+        if (first == 0) {
+            const F sent = from_bits<F>(Bits<F>::sentinel);
+            for (int idx = tid; idx < R * a.k; idx += kTaskThreads) a.part[(size_t)r0 * a.k + idx] = sent;
+        }
+        if (tid == 0) s_next = pre;
+        __syncthreads();
+        tk = s_next;
+        __syncthreads();
+    }
+}
+
+template <typename F>
+hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, QueueState &queues,
+                       hipStream_t stream) {
+    BlockPassArgs<F> a;
+    a.ntasks = bf.ntasks;
+    a.k = bf.k;
+    a.max_rows = bf.max_rows;
+    a.task = bf.task;
+    a.start = bf.start;
+    a.rptr = bf.rptr;
+    a.rlist = bf.rlist;
+    a.moff = bf.moff;
+    a.M = static_cast<const F *>(bf.M);
+    a.cptr = bf.cptr;
+    a.cslot = bf.cslot;
+    a.perm = bf.perm;
+    a.b = static_cast<const F *>(bf.b);
+    a.y = static_cast<F *>(bf.y);
+    a.z = static_cast<F *>(bf.z);
+    a.part = static_cast<F *>(bf.part);
+    a.x = bf.x;
+    a.status = bf.status;
+    a.xmode = bf.xmode;
+    queues.next_launch(a.qa, grid, bf.ntasks);
+    void *args[] = {&a};
+    const size_t lds = block_pass_lds(sizeof(F), bf.max_rows);
+    return hipLaunchKernel(block_pass_kernel(sizeof(F), forward), dim3(grid), dim3(kTaskThreads), args,
+                           lds, stream);
+}
+
+}  // namespace
+
+size_t block_pass_lds(int float_bytes, int max_rows) {
+    return (size_t)(max_rows + kTaskThreads) * kBlockKT * (size_t)float_bytes;
+}
+
+const void *block_pass_kernel(int float_bytes, bool forward) {
+    if (float_bytes == 4)
+        return forward ? reinterpret_cast<const void *>(&block_forward<float, kBlockKT>)
+                       : reinterpret_cast<const void *>(&block_backward<float, kBlockKT>);
+    return forward ? reinterpret_cast<const void *>(&block_forward<double, kBlockKT>)
+                   : reinterpret_cast<const void *>(&block_backward<double, kBlockKT>);
+}
+
+hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffers &buf, int grid,
+                             QueueState &queues, hipStream_t stream) {
+    return float_bytes == 4 ? launch_impl<float>(forward, buf, grid, queues, stream)
+                            : launch_impl<double>(forward, buf, grid, queues, stream);
+}
+
+}  // namespace cholespy_amd

@@ -3,11 +3,12 @@
 // x[perm[r]] = z[r] (kernels/kernels.cu:64-131 and 133-202).
 //
 // Work assignment. Rows are visited in level order (`order`, the sorted
-// `levels` array of the analysis). Ticket t goes to wavefront t mod W of a
-// co-resident persistent grid of W wavefronts; a row's dependencies are in
-// strictly lower levels, so they have smaller tickets and their owners run
-// ahead: the schedule cannot deadlock, and no ticket counter is needed (the
-// reference takes one atomicAdd per row, kernels/kernels.cu:73-75).
+// `levels` array of the analysis). Wavefronts of a persistent grid take
+// chunks of kRowChunk consecutive rows from the ticket queues of common.hpp
+// (the reference takes one atomicAdd per row, kernels/kernels.cu:73-75); a
+// row's dependencies are in strictly lower levels, i.e. earlier tickets held
+// by running wavefronts or earlier rows of the same chunk, so the schedule
+// cannot deadlock.
 //
 // Lane layout for k right-hand sides. kp = min(64, next_pow2(k)) lanes form
 // a group that covers the k columns of one nonzero (a lane covers columns t
@@ -46,6 +47,7 @@ struct RowPassArgs {
     F *z;
     F *x;
     int *status;
+    QueueArgs qa;
 };
 
 constexpr int kUnroll = 4;   // nonzeros per group with loads in flight together
@@ -54,8 +56,9 @@ template <typename F, bool LOWER, int CPL>
 __global__ __launch_bounds__(kRowBlock) void row_pass(RowPassArgs<F> a) {
     using U = typename Bits<F>::U;
     const int lane = threadIdx.x & (kWave - 1);
-    const int waves = gridDim.x * (kRowBlock / kWave);
     const int w = blockIdx.x * (kRowBlock / kWave) + (threadIdx.x / kWave);
+    const int qid = w % a.qa.nq;
+    const int nchunks = (a.n + kRowChunk - 1) / kRowChunk;
     const int kp = 1 << a.kp_log2;
     const int groups = kWave >> a.kp_log2;
     const int g = lane >> a.kp_log2;
@@ -63,7 +66,13 @@ __global__ __launch_bounds__(kRowBlock) void row_pass(RowPassArgs<F> a) {
     const int k = a.k;
     F *dep = LOWER ? a.y : a.z;   // buffer this pass publishes to and waits on
 
-    for (int tk = w; tk < a.n; tk += waves) {
+    int next = lane == 0 ? take_ticket(a.qa, qid, nchunks) : 0;
+    next = __shfl(next, 0, kWave);
+    while (next >= 0) {
+      const int chunk = next;
+      int pre = lane == 0 ? take_ticket(a.qa, qid, nchunks) : 0;   // in flight during the chunk
+      const int tk_end = min(a.n, (chunk + 1) * kRowChunk);
+      for (int tk = chunk * kRowChunk; tk < tk_end; ++tk) {
         const int row = a.order[tk];
         const int rs = a.rowptr[row], re = a.rowptr[row + 1];
         const int lo = LOWER ? rs : rs + 1;
@@ -129,6 +138,8 @@ __global__ __launch_bounds__(kRowBlock) void row_pass(RowPassArgs<F> a) {
                 }
             }
         }
+      }
+      next = __shfl(pre, 0, kWave);
     }
 }
 
@@ -142,7 +153,7 @@ const void *pick(bool lower, int k) {
 }
 
 template <typename F>
-hipError_t launch(bool lower, const RowPassBuffers &bf, int grid, hipStream_t stream) {
+hipError_t launch(bool lower, const RowPassBuffers &bf, int grid, QueueState &queues, hipStream_t stream) {
     RowPassArgs<F> a;
     a.n = bf.n;
     a.k = bf.k;
@@ -159,9 +170,9 @@ hipError_t launch(bool lower, const RowPassBuffers &bf, int grid, hipStream_t st
     a.z = static_cast<F *>(bf.z);
     a.x = static_cast<F *>(bf.x);
     a.status = bf.status;
+    queues.next_launch(a.qa, grid * (kRowBlock / kWave), (bf.n + kRowChunk - 1) / kRowChunk);
     void *args[] = {&a};
-    return hipLaunchCooperativeKernel(pick<F>(lower, bf.k), dim3(grid), dim3(kRowBlock), args, 0,
-                                      stream);
+    return hipLaunchKernel(pick<F>(lower, bf.k), dim3(grid), dim3(kRowBlock), args, 0, stream);
 }
 
 }  // namespace
@@ -171,9 +182,9 @@ const void *row_pass_kernel(int float_bytes, bool lower, int k) {
 }
 
 hipError_t launch_row_pass(int float_bytes, bool lower, const RowPassBuffers &buf, int grid,
-                           hipStream_t stream) {
-    return float_bytes == 4 ? launch<float>(lower, buf, grid, stream)
-                            : launch<double>(lower, buf, grid, stream);
+                           QueueState &queues, hipStream_t stream) {
+    return float_bytes == 4 ? launch<float>(lower, buf, grid, queues, stream)
+                            : launch<double>(lower, buf, grid, queues, stream);
 }
 
 }  // namespace cholespy_amd

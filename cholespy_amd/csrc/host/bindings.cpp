@@ -296,6 +296,50 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_n_levels", [](const Class &self, bool lower) { return self.n_levels(lower); },
              py::arg("lower"))
         .def("_check", [](const Class &self) { self.check(); })
+        .def("_set_path", [](Class &self, int path) { self.set_path(path); }, py::arg("path"))
+        .def(
+            "_solve_pass",
+            [](Class &self, int pass, py::object b_o, py::object x_o) {
+                DLView b(b_o), x(x_o);
+                if (b.device() != Dev::GPU || x.device() != Dev::GPU || !b.c_contig() || !x.c_contig() ||
+                    b.dtype().code != dl::kFloat || b.dtype().bits != 8 * sizeof(Float) ||
+                    x.dtype().code != dl::kFloat || x.dtype().bits != 8 * sizeof(Float))
+                    throw py::type_error("_solve_pass(): b and x must be C-contiguous GPU arrays of the solver dtype.");
+                if (b.ndim() != x.ndim() || b.shape(0) != x.shape(0) || b.shape(0) != self.n() ||
+                    (b.ndim() == 2 && b.shape(1) != x.shape(1)))
+                    throw std::invalid_argument("x and b should have the same dimensions.");
+                self.solve_pass(pass, b.ndim() == 2 ? (int)b.shape(1) : 1, b.data(), x.data());
+            },
+            py::arg("pass_"), py::arg("b"), py::arg("x"))
+        .def(
+            "_solve_refined",
+            [](Class &self, py::object b_o, py::object x_o, int iterations) {
+                DLView b(b_o), x(x_o);
+                const auto good = [](const DLView &v) {
+                    return v.dtype().code == dl::kFloat && v.dtype().bits == 64 && v.dtype().lanes == 1 &&
+                           v.c_contig() && v.device() == Dev::GPU;
+                };
+                if (!good(b) || !good(x) || x.readonly())
+                    throw py::type_error("_solve_refined(): b and x must be C-contiguous float64 GPU arrays.");
+                if (b.ndim() != 1 && b.ndim() != 2)
+                    throw std::invalid_argument("Expected 1D or 2D tensors as input.");
+                if (b.ndim() != x.ndim() || b.shape(0) != x.shape(0) ||
+                    (b.ndim() == 2 && b.shape(1) != x.shape(1)))
+                    throw std::invalid_argument("x and b should have the same dimensions.");
+                if (b.shape(0) != self.n()) throw std::invalid_argument("b and x should have n_rows rows.");
+                self.solve_refined(b.ndim() == 2 ? (int)b.shape(1) : 1, static_cast<const double *>(b.data()),
+                                   static_cast<double *>(x.data()), iterations);
+            },
+            py::arg("b"), py::arg("x"), py::arg("iterations") = 2)
+        .def("_plan_info",
+             [](const Class &self) {
+                 const std::vector<int64_t> v = self.plan_info();
+                 const char *names[] = {"blocks", "m_entries", "levels_fwd", "levels_bwd",
+                                        "fwd_tasks", "bwd_tasks", "slots", "max_rows"};
+                 py::dict d;
+                 for (size_t i = 0; i < v.size(); ++i) d[names[i]] = v[i];
+                 return d;
+             })
         .def_property_readonly("_n", [](const Class &self) { return self.n(); })
         .def_property_readonly("_is_cpu", [](const Class &self) { return self.is_cpu(); });
 }
@@ -316,4 +360,6 @@ PYBIND11_MODULE(_cholespy_core, m) {
                      cholespy_shutdown();
                  }));
     m.attr("__version__") = "2.2.2";
+    m.attr("_PATH_ROWS") = (int)CHOLESPY_PATH_ROWS;
+    m.attr("_PATH_BLOCKS") = (int)CHOLESPY_PATH_BLOCKS;
 }

@@ -30,6 +30,8 @@ CholeskySolver<Float>::CholeskySolver(int n_rows, int64_t nnz, const int *ii, co
                                                      factor_.nnz()));
         throw_on_error(cholespy_solver_analyze(gpu_, 1, &n_levels_lower_));
         throw_on_error(cholespy_solver_analyze(gpu_, 0, &n_levels_upper_));
+        throw_on_error(cholespy_solver_upload_matrix(gpu_, A_.colptr.data(), A_.rowidx.data(),
+                                                     A_.val.data(), A_.nnz()));
     } catch (...) {
         cholespy_solver_destroy(gpu_);
         gpu_ = nullptr;
@@ -77,6 +79,34 @@ void CholeskySolver<Float>::levels(bool lower, std::vector<int> &order,
 template <typename Float>
 void CholeskySolver<Float>::check() const {
     if (gpu_) throw_on_error(cholespy_solver_check(gpu_));
+}
+
+template <typename Float>
+void CholeskySolver<Float>::solve_pass(int pass, int n_rhs, const void *b, void *x) {
+    if (!gpu_) throw std::invalid_argument("Passes only exist for GPU solvers.");
+    if (n_rhs > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
+    throw_on_error(cholespy_solver_solve_pass(gpu_, pass, n_rhs, b, x, nullptr));
+}
+
+template <typename Float>
+void CholeskySolver<Float>::solve_refined(int n_rhs, const double *b, double *x, int iterations) {
+    if (!gpu_) throw std::invalid_argument("Refined solves only exist for GPU solvers.");
+    if (n_rhs > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
+    throw_on_error(cholespy_solver_solve_refined(gpu_, n_rhs, b, x, iterations, nullptr));
+}
+
+template <typename Float>
+void CholeskySolver<Float>::set_path(int path) {
+    if (!gpu_) throw std::invalid_argument("Solve paths only exist for GPU solvers.");
+    throw_on_error(cholespy_solver_set_path(gpu_, path));
+}
+
+template <typename Float>
+std::vector<int64_t> CholeskySolver<Float>::plan_info() const {
+    if (!gpu_) throw std::invalid_argument("Block plans only exist for GPU solvers.");
+    std::vector<int64_t> info(8, 0);
+    throw_on_error(cholespy_solver_plan_info(gpu_, info.data(), (int)info.size()));
+    return info;
 }
 
 template class CholeskySolver<float>;

@@ -39,6 +39,14 @@ public:
     void levels(bool lower, std::vector<int> &order, std::vector<int> &row_level) const;
     // Waits for the device and raises if a kernel reported a timeout.
     void check() const;
+    // One pass of solve_cuda (0 = forward, 1 = backward).
+    void solve_pass(int pass, int n_rhs, const void *b, void *x);
+    // FP32 solve + FP64 refinement on the GPU (float32 solvers): b, x are
+    // device addresses of row-major double arrays.
+    void solve_refined(int n_rhs, const double *b, double *x, int iterations);
+    // Selects the GPU schedule (CHOLESPY_PATH_ROWS / CHOLESPY_PATH_BLOCKS).
+    void set_path(int path);
+    std::vector<int64_t> plan_info() const;
 
 private:
     int n_ = 0;

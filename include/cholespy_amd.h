@@ -43,6 +43,17 @@ typedef enum cholespy_status {
 
 typedef struct cholespy_gpu_solver cholespy_gpu_solver; /* opaque */
 
+/* Solve schedules (cholespy_solver_set_path). */
+enum {
+    /* Level-scheduled row-per-wavefront passes over the row-compressed L and
+     * L' in level order: the reference's algorithm (kernels/kernels.cu:64-202)
+     * laid out for gfx950. The "plain" baseline. */
+    CHOLESPY_PATH_ROWS = 0,
+    /* Supernodal partitioned-inverse passes: one dense product per block of
+     * the elimination tree (default). */
+    CHOLESPY_PATH_BLOCKS = 1
+};
+
 /* Lazy, idempotent device initialisation. Replaces init_cuda()
  * (src/cuda_driver.cpp:58-136). Selects the calling thread's current HIP
  * device and records its compute-unit count for persistent launches. */
@@ -109,6 +120,38 @@ int cholespy_solver_solve_f32(cholespy_gpu_solver *solver, int n_rhs, const floa
                               float *x, void *stream);
 int cholespy_solver_solve_f64(cholespy_gpu_solver *solver, int n_rhs, const double *b,
                               double *x, void *stream);
+
+/* One triangular pass of a solve: pass 0 = forward (reads b, fills the
+ * handle's intermediate y), pass 1 = backward (reads y, writes x). Running
+ * pass 0 then pass 1 with the same n_rhs equals one solve. Replaces
+ * launch_kernel(lower, x) (src/cholesky_solver.cpp:378-413); used to time
+ * the two kernels separately. b is ignored by pass 1 and x by pass 0 (either
+ * may then be NULL). */
+int cholespy_solver_solve_pass(cholespy_gpu_solver *solver, int pass, int n_rhs, const void *b,
+                               void *x, void *stream);
+
+/* Uploads A itself (its lower triangle, compressed columns, rows
+ * ascending, FP64), as the refinement residual needs it. No counterpart in
+ * the reference, whose GPU path keeps only L. */
+int cholespy_solver_upload_matrix(cholespy_gpu_solver *solver, const int *colptr,
+                                  const int *rowidx, const double *val, int64_t nnz);
+
+/* Mixed-precision solve for a float32 handle: x (double, row-major n x
+ * n_rhs, device) := A^{-1} b (double) by an FP32 block solve followed by
+ * `iterations` FP64 refinement steps (r = b - A x in FP64, FP32 solve of r,
+ * x += correction in FP64). Asynchronous on `stream`. No counterpart in the
+ * reference (its CholeskySolverF solves in FP32 only). */
+int cholespy_solver_solve_refined(cholespy_gpu_solver *solver, int n_rhs, const double *b,
+                                  double *x, int iterations, void *stream);
+
+/* Selects the schedule used by later solves (default CHOLESPY_PATH_BLOCKS).
+ * No counterpart in the reference, which has the row schedule only. */
+int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);
+
+/* Block-plan statistics: info[0..count) = {blocks, dense entries of all
+ * M_B, forward block levels, backward block levels, forward tasks, backward
+ * tasks, partial slots, largest m + |R|}. No counterpart in the reference. */
+int cholespy_solver_plan_info(const cholespy_gpu_solver *solver, int64_t *info, int count);
 
 /* Reads and clears the handle's device status word (the bounded-wait
  * watchdog of the solve and analysis kernels). Synchronises the device.

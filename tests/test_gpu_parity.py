@@ -10,8 +10,11 @@ import pytest
 
 torch = pytest.importorskip("torch")
 
+import cholespy_amd
 from cholespy_amd import CholeskySolverD, CholeskySolverF, MatrixType, workloads
 from oracle import oracle
+
+PATHS = {"rows": cholespy_amd._core._PATH_ROWS, "blocks": cholespy_amd._core._PATH_BLOCKS}
 
 pytestmark = pytest.mark.gpu
 
@@ -47,11 +50,13 @@ def test_levels_bit_exact(case):
         assert solver._n_levels(lower) == mx + 1
 
 
+@pytest.mark.parametrize("path", list(PATHS))
 @pytest.mark.parametrize("case", list(CASES))
 @pytest.mark.parametrize("cls,dtype", [(CholeskySolverF, np.float32), (CholeskySolverD, np.float64)])
-def test_solve_matches_oracle(case, cls, dtype):
+def test_solve_matches_oracle(case, cls, dtype, path):
     n, values, idx = CASES[case]()
     solver = _gpu_solver(cls, n, values, idx)
+    solver._set_path(PATHS[path])
     fac = solver._factor()
     rng = np.random.default_rng(1)
     for k in (1, 3, 4, 16, 64, 65, 128):
@@ -71,12 +76,14 @@ def test_solve_matches_oracle(case, cls, dtype):
 
 
 def test_repeated_solves_and_rhs_change():
-    """Work buffers are reused across solves and re-laid-out when k changes."""
+    """Work buffers are reused across solves, re-laid-out when k changes and
+    re-initialised when the schedule changes."""
     n, values, idx = CASES["ico5"]()
     solver = _gpu_solver(CholeskySolverF, n, values, idx)
     fac = solver._factor()
     rng = np.random.default_rng(2)
-    for k in (3, 3, 1, 128, 3, 7, 7):
+    for i, k in enumerate((3, 3, 1, 128, 3, 7, 7, 3, 3)):
+        solver._set_path(PATHS["rows"] if i in (5, 7) else PATHS["blocks"])
         b = rng.standard_normal((n, k)).astype(np.float32)
         bt = torch.tensor(b, device=DEV)
         xt = torch.zeros_like(bt)
@@ -113,8 +120,11 @@ def test_full_size_mesh_smoothing_residual():
     solve's relative residual."""
     n, values, idx, rhs = workloads.smoothing_system(7, 10.0, seed=0)
     A = oracle.symmetric_from_lower(n, *oracle.normalized_lower_csc(n, idx[0], idx[1], values, "COO"))
-    for cls, dtype, tol in ((CholeskySolverD, np.float64, 1e-12), (CholeskySolverF, np.float32, 1e-5)):
+    for cls, dtype, tol, path in ((CholeskySolverD, np.float64, 1e-12, "blocks"),
+                                  (CholeskySolverF, np.float32, 1e-5, "blocks"),
+                                  (CholeskySolverD, np.float64, 1e-12, "rows")):
         solver = _gpu_solver(cls, n, values, idx)
+        solver._set_path(PATHS[path])
         b = rhs.astype(dtype)
         bt = torch.tensor(b, device=DEV)
         xt = torch.zeros_like(bt)
@@ -126,3 +136,42 @@ def test_full_size_mesh_smoothing_residual():
         rel = np.linalg.norm(x - host) / np.linalg.norm(host)
         res = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
         assert rel < tol and res < tol, (dtype, rel, res)
+
+
+@pytest.mark.parametrize("case", ["ico5", "grid64"])
+def test_refined_reaches_fp64_accuracy(case):
+    """FP32 block solve + FP64 refinement: <= 1e-10 relative error against
+    the FP64 host solve on the same factor after 2 steps, and the error
+    shrinks with each step."""
+    n, values, idx = CASES[case]()
+    solver = _gpu_solver(CholeskySolverF, n, values, idx)
+    rng = np.random.default_rng(5)
+    b = rng.standard_normal((n, 3))
+    host = oracle.host_solve(solver._factor(), b)
+    bt = torch.tensor(b, device=DEV)
+    errs = []
+    for it in (0, 1, 2):
+        xt = torch.zeros_like(bt)
+        solver._solve_refined(bt, xt, it)
+        torch.cuda.synchronize()
+        solver._check()
+        errs.append(np.linalg.norm(xt.cpu().numpy() - host) / np.linalg.norm(host))
+    assert errs[2] <= 1e-10, errs
+    assert errs[1] < errs[0] * 1e-2, errs
+
+
+def test_single_passes_compose_to_a_solve():
+    """solve_pass(0) then solve_pass(1) equals solve() (launch_kernel(lower)
+    then launch_kernel(upper), src/cholesky_solver.cpp:428-431)."""
+    n, values, idx = CASES["ico5"]()
+    for cls, dtype in ((CholeskySolverF, torch.float32), (CholeskySolverD, torch.float64)):
+        solver = _gpu_solver(cls, n, values, idx)
+        b = torch.rand(n, 3, device=DEV, dtype=dtype)
+        x1 = torch.zeros_like(b)
+        x2 = torch.zeros_like(b)
+        solver.solve(b, x1)
+        solver._solve_pass(0, b, x2)
+        solver._solve_pass(1, b, x2)
+        torch.cuda.synchronize()
+        solver._check()
+        assert torch.equal(x1, x2)
