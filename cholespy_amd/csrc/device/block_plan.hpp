@@ -23,7 +23,7 @@ struct BlockPlanOptions {
     int max_block = 256;    // rows per block (dense inverse is m x m)
     int leaf_max = 16;      // whole subtrees up to this size become one block
     int fwd_rows = 64;      // rows of M_B per forward task
-    int bwd_cols = 64;      // columns of M_B per backward task
+    int bwd_cols = 32;      // columns of M_B per backward task
 };
 
 struct BlockPlan {

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -125,6 +126,7 @@ struct cholespy_gpu_solver {
     // backward/forward, [4..5] block pass backward/forward.
     unsigned long long *qctr = nullptr;
     QueueState queues[6];
+    unsigned long long *trace[2] = {nullptr, nullptr};   // block pass timelines [backward, forward]
     std::vector<int> host_levels[2], host_row_level[2];
 };
 
@@ -201,6 +203,8 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->perm);
     dev_free(s->lvl1);
     dev_free(s->qctr);
+    dev_free(s->trace[0]);
+    dev_free(s->trace[1]);
     dev_free(s->lrowptr);
     dev_free(s->lcols);
     dev_free(s->lvals);
@@ -294,8 +298,11 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     s->uploaded = true;
     s->analysed[0] = s->analysed[1] = false;
 
-    // Block plan for the partitioned-inverse path.
+    // Block plan for the partitioned-inverse path. CHOLESPY_MAX_BLOCK and
+    // CHOLESPY_LEAF_MAX override the block-size limits (tuning sweeps).
     BlockPlanOptions opt;
+    if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
     BlockPlan P = build_block_plan(n, colptr, rowidx, val, opt);
     DeviceBlockPlan &d = s->plan;
     d.release();
@@ -508,6 +515,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
         bf.ntasks = forward ? d.n_fwd_tasks : d.n_bwd_tasks;
         bf.task = forward ? d.ftask : d.btask;
         bf.M = forward ? d.mfwd : d.mbwd;
+        bf.trace = s->trace[forward ? 1 : 0];
         int &grid = s->grid_blocks[forward ? 1 : 0];
         if (grid == 0) {
             const size_t lds = block_pass_lds(fbytes, d.max_rows);
@@ -630,6 +638,43 @@ int cholespy_solver_solve_pass(cholespy_gpu_solver *s, int pass, int n_rhs, cons
     if (pass != 0 && pass != 1) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "pass must be 0 or 1");
     if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
     return solve_impl(s, s->fbytes, n_rhs, b ? b : x, x ? x : const_cast<void *>(b), stream, pass);
+}
+
+int cholespy_entry_gradient(int64_t nnz, const int *rows, const int *cols, int n_rhs, const void *lam,
+                            const void *x, int vec_bytes, void *grad, int grad_bytes, void *stream) {
+    if (nnz < 0 || n_rhs < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative size");
+    if ((vec_bytes != 4 && vec_bytes != 8) || (grad_bytes != 4 && grad_bytes != 8))
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "element sizes must be 4 or 8");
+    if (nnz == 0) return CHOLESPY_OK;
+    if (!rows || !cols || !grad || (n_rhs > 0 && (!lam || !x)))
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null pointer");
+    HIP_TRY(launch_entry_gradient(nnz, rows, cols, n_rhs, lam, x, vec_bytes, grad, grad_bytes,
+                                  static_cast<hipStream_t>(stream)));
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_set_trace(cholespy_gpu_solver *s, int enable) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    dev_free(s->trace[0]);
+    dev_free(s->trace[1]);
+    if (!enable) return CHOLESPY_OK;
+    const size_t words[2] = {4 * (size_t)std::max(1, s->plan.n_bwd_tasks),
+                             4 * (size_t)std::max(1, s->plan.n_fwd_tasks)};
+    for (int i = 0; i < 2; ++i) {
+        HIP_TRY(dev_alloc(&s->trace[i], words[i]));
+        HIP_TRY(hipMemset(s->trace[i], 0, words[i] * sizeof(unsigned long long)));
+    }
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_get_trace(const cholespy_gpu_solver *s, int forward, uint64_t *out, int64_t words) {
+    if (!s || !out) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    const unsigned long long *t = s->trace[forward ? 1 : 0];
+    if (!t) return fail(CHOLESPY_ERR_STATE, "tracing is off");
+    const int64_t have = 4 * (int64_t)(forward ? s->plan.n_fwd_tasks : s->plan.n_bwd_tasks);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(out, t, (size_t)std::min(have, words) * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    return CHOLESPY_OK;
 }
 
 int cholespy_solver_set_path(cholespy_gpu_solver *s, int path) {

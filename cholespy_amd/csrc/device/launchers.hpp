@@ -67,6 +67,7 @@ struct BlockPassBuffers {
     // Backward epilogue: 0 = x (solver precision) = z; 1 = x (double) = z;
     // 2 = x (double) += z (refinement update).
     int xmode = 0;
+    unsigned long long *trace = nullptr;   // optional task timeline (4 words per task)
 };
 size_t block_pass_lds(int float_bytes, int max_rows);
 const void *block_pass_kernel(int float_bytes, bool forward);
@@ -77,6 +78,11 @@ hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffe
 hipError_t launch_cast_to_f32(const double *in, float *out, size_t count, hipStream_t stream);
 hipError_t launch_residual_f64(int n, int k, const int *rowptr, const int *cols, const double *vals,
                                const double *b, const double *x, float *r32, hipStream_t stream);
+
+// dL/d(stored entries of A) from the adjoint lam and the solution x (grad.hip).
+hipError_t launch_entry_gradient(int64_t nnz, const int *rows, const int *cols, int k, const void *lam,
+                                 const void *x, int vec_bytes, void *grad, int grad_bytes,
+                                 hipStream_t stream);
 
 // Fills n words of `bytes` (4 or 8) with the sentinel of that width.
 hipError_t launch_fill_sentinel(void *p, size_t n, int bytes, hipStream_t stream);

@@ -14,10 +14,12 @@
 // while the current task runs), so every awaited task is running or done and
 // the schedule cannot deadlock.
 //
-// Each task is one workgroup of 256 threads: 64 rows (forward) or 64 columns
-// (backward) x 4 segments of the reduction dimension, so every M load is a
-// 256-byte coalesced wavefront access, and the 4 segment sums are combined in
-// LDS. Right-hand sides are processed in tiles of KT columns held in LDS.
+// Each task is one workgroup of 256 threads: 64 rows x 4 column segments
+// (forward; each M load is a 256-byte wavefront access down one column) or
+// 32 columns x 8 row segments (backward; 128-byte pieces of M_B's rows), so
+// loads are coalesced; the segment sums are combined in LDS. Every thread
+// keeps 16 M loads in flight. Right-hand sides are processed in tiles of KT
+// columns held in LDS.
 //
 // Sentinel hand-off as in common.hpp. The forward pass resets z for its rows,
 // the backward pass resets the partial slots of its block, so the next solve
@@ -30,9 +32,13 @@ namespace cholespy_amd {
 namespace {
 
 constexpr int kTaskThreads = 256;
-constexpr int kLanes = 64;      // rows / columns per task
-constexpr int kSegs = 4;        // segments of the reduction dimension
+constexpr int kLanes = 64;      // forward: rows of M_B per task
+constexpr int kSegs = 4;        // forward: segments of the column range
+constexpr int kBLanes = 32;     // backward: columns of M_B per task (128-byte row pieces)
+constexpr int kBSegs = 8;       // backward: segments of the row range
 constexpr int kGatherUnroll = 8;
+constexpr int kMU = 16;         // M loads issued together per thread
+constexpr int kPairs = 4;       // forward gather: (row, column) pairs walked together
 
 template <typename F>
 struct BlockPassArgs {
@@ -52,6 +58,9 @@ struct BlockPassArgs {
     int *status;
     int xmode;
     QueueArgs qa;
+    // Optional task timeline (nullptr = off): per task, 4 words =
+    // {start, after the gather phase, end, workgroup id} in wall-clock ticks (100 MHz).
+    unsigned long long *trace;
 };
 
 // Ticket handling per workgroup: thread 0 takes tickets, the LDS word s_next
@@ -69,31 +78,9 @@ __device__ __forceinline__ int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {
     return rho < m ? rho * (rho + 1) / 2 + c : m * (m + 1) / 2 + (rho - m) * m + c;
 }
 
-// Sum of the partials in slots cslot[p0..p1) for column col, in slot order.
-template <typename F>
-__device__ __forceinline__ F gather_partials(const BlockPassArgs<F> &a, int p0, int p1, int col) {
-    using U = typename Bits<F>::U;
-    F s = F(0);
-    for (int p = p0; p < p1; p += kGatherUnroll) {
-        U u[kGatherUnroll];
-        int slot[kGatherUnroll];
-#pragma unroll
-        for (int j = 0; j < kGatherUnroll; ++j) {
-            slot[j] = p + j < p1 ? a.cslot[p + j] : -1;
-            u[j] = slot[j] >= 0 ? load_bits(&a.part[(size_t)slot[j] * a.k + col]) : U(0);
-        }
-#pragma unroll
-        for (int j = 0; j < kGatherUnroll; ++j) {
-            F v = from_bits<F>(u[j]);
-            if (u[j] == Bits<F>::sentinel) v = wait_value(&a.part[(size_t)slot[j] * a.k + col], a.status);
-            s += v;
-        }
-    }
-    return s;
-}
-
 template <typename F, int KT>
 __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a) {
+    using U = typename Bits<F>::U;
     extern __shared__ __align__(16) unsigned char smem[];
     F *rv = reinterpret_cast<F *>(smem);                   // max_rows x KT (only m used)
     F *red = rv + (size_t)a.max_rows * KT;                 // 256 x KT
@@ -105,6 +92,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
     while (tk >= 0) {
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
+        if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const int blk = a.task[3 * tk], first = a.task[3 * tk + 1], last = a.task[3 * tk + 2];
         const int a0 = a.start[blk];
         const int m = a.start[blk + 1] - a0;
@@ -113,17 +101,58 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const F *M = a.M + a.moff[blk];
         for (int t0 = 0; t0 < a.k; t0 += KT) {
             const int kt = min(KT, a.k - t0);
-            for (int idx = tid; idx < m * KT; idx += kTaskThreads) {
-                const int i = idx / KT, t = idx % KT;
-                F v = F(0);
-                if (t < kt) {
-                    const int row = a0 + i, col = t0 + t;
-                    v = a.b[(size_t)a.perm[row] * a.k + col] +
-                        gather_partials(a, a.cptr[row], a.cptr[row + 1], col);
+            // r_B: each thread owns up to kPairs (row, column) pairs and
+            // walks their partial-slot lists together, 4 slots per pair per
+            // round, so up to 4 * kPairs slot loads are in flight at once.
+            for (int base = tid; base < m * KT; base += kTaskThreads * kPairs) {
+                int p0[kPairs], p1[kPairs], col[kPairs];
+                F v[kPairs];
+#pragma unroll
+                for (int q = 0; q < kPairs; ++q) {
+                    const int idx = base + q * kTaskThreads;
+                    const int i = idx / KT, t = idx % KT;
+                    p0[q] = p1[q] = 0;
+                    col[q] = t0 + t;
+                    v[q] = F(0);
+                    if (idx < m * KT && t < kt) {
+                        const int row = a0 + i;
+                        v[q] = a.b[(size_t)a.perm[row] * a.k + col[q]];
+                        p0[q] = a.cptr[row];
+                        p1[q] = a.cptr[row + 1];
+                    }
                 }
-                rv[idx] = v;
+                for (int r = 0;; r += 4) {
+                    bool any = false;
+                    U u[kPairs][4];
+                    int sl[kPairs][4];
+#pragma unroll
+                    for (int q = 0; q < kPairs; ++q)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const int p = p0[q] + r + j;
+                            sl[q][j] = p < p1[q] ? a.cslot[p] : -1;
+                            u[q][j] = sl[q][j] >= 0 ? load_bits(&a.part[(size_t)sl[q][j] * a.k + col[q]]) : U(0);
+                            any |= sl[q][j] >= 0;
+                        }
+                    if (!any) break;
+#pragma unroll
+                    for (int q = 0; q < kPairs; ++q)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            F x = from_bits<F>(u[q][j]);
+                            if (sl[q][j] >= 0 && u[q][j] == Bits<F>::sentinel)
+                                x = wait_value(&a.part[(size_t)sl[q][j] * a.k + col[q]], a.status);
+                            v[q] += x;
+                        }
+                }
+#pragma unroll
+                for (int q = 0; q < kPairs; ++q) {
+                    const int idx = base + q * kTaskThreads;
+                    if (idx < m * KT) rv[idx] = v[q];
+                }
             }
             __syncthreads();
+            if (a.trace && tid == 0 && t0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
             const int rho = first + lr;
             F acc[KT];
 #pragma unroll
@@ -132,8 +161,26 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                 const int cend = rho < m ? rho + 1 : m;
                 const int cq = (m + kSegs - 1) / kSegs;
                 const int cb = seg * cq, ce = min(cend, cb + cq);
-                for (int c = cb; c < ce; ++c) {
-                    const F mv = M[fwd_off(rows, c, rho)];
+                // Column c of the packed column-major M_B starts (rows - c)
+                // entries after column c - 1: walk the offsets incrementally
+                // and keep kMU loads in flight before consuming them.
+                int64_t o = fwd_off(rows, cb, rho);
+                int c = cb;
+                for (; c + kMU <= ce; c += kMU) {
+                    F mv[kMU];
+#pragma unroll
+                    for (int j = 0; j < kMU; ++j) {
+                        mv[j] = M[o];
+                        o += rows - (c + j) - 1;
+                    }
+#pragma unroll
+                    for (int j = 0; j < kMU; ++j)
+#pragma unroll
+                        for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], rv[(c + j) * KT + t], acc[t]);
+                }
+                for (; c < ce; ++c) {
+                    const F mv = M[o];
+                    o += rows - c - 1;
 #pragma unroll
                     for (int t = 0; t < KT; ++t) acc[t] = fma(mv, rv[c * KT + t], acc[t]);
                 }
@@ -160,6 +207,10 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
             }
             __syncthreads();
         }
+        if (a.trace && tid == 0) {
+            a.trace[4 * (size_t)tk + 2] = wall_clock64();
+            a.trace[4 * (size_t)tk + 3] = blockIdx.x;
+        }
         if (tid == 0) s_next = pre;
         __syncthreads();
         tk = s_next;
@@ -175,12 +226,13 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
     F *red = vs + (size_t)a.max_rows * KT;                 // 256 x KT
     __shared__ int s_next;
     const int tid = threadIdx.x;
-    const int lr = tid & (kLanes - 1), seg = tid / kLanes;
+    const int lr = tid & (kBLanes - 1), seg = tid / kBLanes;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
     while (tk >= 0) {
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
+        if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const int blk = a.task[3 * tk], first = a.task[3 * tk + 1], last = a.task[3 * tk + 2];
         const int a0 = a.start[blk];
         const int m = a.start[blk + 1] - a0;
@@ -219,15 +271,33 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                 }
             }
             __syncthreads();
+            if (a.trace && tid == 0 && t0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
             const int c = first + lr;
             F acc[KT];
 #pragma unroll
             for (int t = 0; t < KT; ++t) acc[t] = F(0);
             if (c < last) {
-                const int rq = (rows + kSegs - 1) / kSegs;
+                const int rq = (rows + kBSegs - 1) / kBSegs;
                 const int rb = max(seg * rq, c), re = min(rows, seg * rq + rq);
-                for (int rho = rb; rho < re; ++rho) {
-                    const F mv = M[bwd_off(m, rho, c)];
+                // Row rho of the packed row-major M_B is min(rho + 1, m)
+                // entries long: walk the offsets incrementally, kMU loads in flight.
+                int64_t o = bwd_off(m, rb, c);
+                int rho = rb;
+                for (; rho + kMU <= re; rho += kMU) {
+                    F mv[kMU];
+#pragma unroll
+                    for (int j = 0; j < kMU; ++j) {
+                        mv[j] = M[o];
+                        o += min(rho + j + 1, m);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kMU; ++j)
+#pragma unroll
+                        for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], vs[(rho + j) * KT + t], acc[t]);
+                }
+                for (; rho < re; ++rho) {
+                    const F mv = M[o];
+                    o += min(rho + 1, m);
 #pragma unroll
                     for (int t = 0; t < KT; ++t) acc[t] = fma(mv, vs[rho * KT + t], acc[t]);
                 }
@@ -241,7 +311,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     if (t >= kt) break;
                     F s = red[lr * KT + t];
 #pragma unroll
-                    for (int g = 1; g < kSegs; ++g) s += red[(g * kLanes + lr) * KT + t];
+                    for (int g = 1; g < kBSegs; ++g) s += red[(g * kBLanes + lr) * KT + t];
                     const int col = t0 + t;
                     publish(&a.z[(size_t)(a0 + c) * a.k + col], s);
                     const size_t o = (size_t)a.perm[a0 + c] * a.k + col;
@@ -259,6 +329,10 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         if (first == 0) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
             for (int idx = tid; idx < R * a.k; idx += kTaskThreads) a.part[(size_t)r0 * a.k + idx] = sent;
+        }
+        if (a.trace && tid == 0) {
+            a.trace[4 * (size_t)tk + 2] = wall_clock64();
+            a.trace[4 * (size_t)tk + 3] = blockIdx.x;
         }
         if (tid == 0) s_next = pre;
         __syncthreads();
@@ -290,6 +364,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.x = bf.x;
     a.status = bf.status;
     a.xmode = bf.xmode;
+    a.trace = bf.trace;
     queues.next_launch(a.qa, grid, bf.ntasks);
     void *args[] = {&a};
     const size_t lds = block_pass_lds(sizeof(F), bf.max_rows);

@@ -296,6 +296,17 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_n_levels", [](const Class &self, bool lower) { return self.n_levels(lower); },
              py::arg("lower"))
         .def("_check", [](const Class &self) { self.check(); })
+        .def("_set_trace", [](Class &self, bool on) { throw_on_error(cholespy_solver_set_trace(self.gpu(), on ? 1 : 0)); },
+             py::arg("on"))
+        .def("_trace",
+             [](Class &self, bool forward) {
+                 const std::vector<int64_t> info = self.plan_info();
+                 const int64_t words = 4 * (forward ? info[4] : info[5]);
+                 py::array_t<uint64_t> out(words);
+                 throw_on_error(cholespy_solver_get_trace(self.gpu(), forward ? 1 : 0, out.mutable_data(), words));
+                 return out;
+             },
+             py::arg("forward"))
         .def("_set_path", [](Class &self, int path) { self.set_path(path); }, py::arg("path"))
         .def(
             "_solve_pass",
@@ -346,7 +357,33 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
 
 }  // namespace
 
+// dL/d(stored entries) on the GPU (grad.hip through the C ABI): rows / cols
+// int32, lam / x row-major float arrays of one dtype, grad float32/64.
+void entry_gradient(py::object rows_o, py::object cols_o, py::object lam_o, py::object x_o,
+                    py::object grad_o) {
+    DLView rows(rows_o), cols(cols_o), lam(lam_o), x(x_o), grad(grad_o);
+    for (const DLView *v : {&rows, &cols, &lam, &x, &grad})
+        if (v->device() != Dev::GPU || !v->c_contig())
+            throw py::type_error("_entry_gradient(): all arrays must be C-contiguous GPU arrays.");
+    if (rows.dtype().code != dl::kInt || rows.dtype().bits != 32 || cols.dtype().code != dl::kInt ||
+        cols.dtype().bits != 32)
+        throw py::type_error("_entry_gradient(): rows and cols must be int32.");
+    if (lam.dtype().code != dl::kFloat || x.dtype().code != dl::kFloat || grad.dtype().code != dl::kFloat ||
+        lam.dtype().bits != x.dtype().bits)
+        throw py::type_error("_entry_gradient(): lam, x and grad must be float arrays (lam, x same dtype).");
+    if (rows.size() != cols.size() || rows.size() != grad.size() || lam.ndim() != x.ndim() ||
+        lam.size() != x.size())
+        throw std::invalid_argument("_entry_gradient(): inconsistent sizes.");
+    const int k = x.ndim() == 2 ? (int)x.shape(1) : 1;
+    throw_on_error(cholespy_entry_gradient(rows.size(), static_cast<const int *>(rows.data()),
+                                           static_cast<const int *>(cols.data()), k, lam.data(), x.data(),
+                                           lam.dtype().bits / 8, grad.data(), grad.dtype().bits / 8,
+                                           nullptr));
+}
+
 PYBIND11_MODULE(_cholespy_core, m) {
+    m.def("_entry_gradient", &entry_gradient, py::arg("rows"), py::arg("cols"), py::arg("lam"),
+          py::arg("x"), py::arg("grad"));
     py::enum_<MatrixType>(m, "MatrixType", doc_matrix_type)
         .value("CSC", MatrixType::CSC)
         .value("CSR", MatrixType::CSR)

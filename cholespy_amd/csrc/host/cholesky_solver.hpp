@@ -5,6 +5,7 @@
 // triangular solves; in CPU mode solves stay on the host in FP64.
 #pragma once
 #include <cstdint>
+#include <stdexcept>
 #include <vector>
 
 #include "../../../include/cholespy_amd.h"
@@ -47,6 +48,11 @@ public:
     // Selects the GPU schedule (CHOLESPY_PATH_ROWS / CHOLESPY_PATH_BLOCKS).
     void set_path(int path);
     std::vector<int64_t> plan_info() const;
+    // The C ABI handle (GPU mode only; throws in CPU mode).
+    cholespy_gpu_solver *gpu() const {
+        if (!gpu_) throw std::invalid_argument("Only GPU solvers have a device handle.");
+        return gpu_;
+    }
 
 private:
     int n_ = 0;

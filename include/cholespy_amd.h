@@ -153,6 +153,27 @@ int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);
  * tasks, partial slots, largest m + |R|}. No counterpart in the reference. */
 int cholespy_solver_plan_info(const cholespy_gpu_solver *solver, int64_t *info, int count);
 
+/* Gradient of a loss with respect to the caller's stored matrix entries
+ * (rows[e], cols[e], e < nnz, device int32 arrays) for x = A^{-1} b, given
+ * x and the adjoint lam = A^{-1} dL/dx (device, row-major n x n_rhs,
+ * vec_bytes = 4 or 8): grad[e] = -sum_t (lam[i,t] x[j,t] + lam[j,t] x[i,t])
+ * for i > j, -sum_t lam[i,t] x[i,t] for i == j, 0 for i < j (the strict
+ * upper triangle is never read: CHOLMOD stype = -1,
+ * src/cholesky_solver.cpp:211-220). grad has grad_bytes = 4 or 8 per
+ * element. No counterpart in the reference (it has no backward). */
+int cholespy_entry_gradient(int64_t nnz, const int *rows, const int *cols, int n_rhs,
+                            const void *lam, const void *x, int vec_bytes, void *grad,
+                            int grad_bytes, void *stream);
+
+/* Task timeline of the block passes (diagnostics; no counterpart in the
+ * reference). With tracing on, every block-pass launch records per task
+ * {start, end of the gather phase, end, workgroup} as 100 MHz wall-clock
+ * ticks; get_trace copies min(words, 4 * tasks) words of the last forward
+ * (forward = 1) or backward pass. */
+int cholespy_solver_set_trace(cholespy_gpu_solver *solver, int enable);
+int cholespy_solver_get_trace(const cholespy_gpu_solver *solver, int forward, uint64_t *out,
+                              int64_t words);
+
 /* Reads and clears the handle's device status word (the bounded-wait
  * watchdog of the solve and analysis kernels). Synchronises the device.
  * Returns CHOLESPY_ERR_TIMEOUT if a kernel gave up waiting since the last
