@@ -14,8 +14,9 @@ Paths timed in the same process, inputs resident in HBM:
 A step is one solve of the 3 right-hand sides; `value` = 3 * steps / time.
 
 roofline: the dominant kernel of the headline path (the FP32 block pass that
-takes longer), timed alone with HIP events on the stream it runs on (the
-null stream); achieved = algorithmic bytes of that pass / its average time.
+takes longer), timed with HIP events recorded between the two passes of
+complete solves on the stream they run on (the null stream); achieved =
+algorithmic bytes of that pass / its average time.
 cpu_baseline: the oracle's FP64 simplicial solve (oracle/oracle.c,
 oracle_host_solve: what cholmod_solve does for a simplicial LL' factor) on
 the same factor, one thread, bounded to ~10 s of work.
@@ -74,11 +75,15 @@ def main():
     ap.add_argument("--level", type=int, default=7, help="icosphere subdivision level")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--workload", choices=["smoothing", "batch"], default="smoothing")
+    ap.add_argument("--batch-count", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the solve phase is single-device); "
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import torch
 
@@ -171,11 +176,24 @@ def main():
     pb = pass_bytes(s_bytes, n, nnz_l, k)
     rhs_pass = b32 if headline == "refined" else b64
     out_pass = x32 if headline == "refined" else x64
-    pass_ms = []
-    for p in (0, 1):
-        if p == 1:
-            solver._solve_pass(0, rhs_pass, out_pass)   # y for the backward pass
-        pass_ms.append(time_events(lambda: solver._solve_pass(p, rhs_pass, out_pass), 50))
+    # Each pass timed inside complete solves (forward then backward, events
+    # between them), so every dependency wait of the real schedule is paid.
+    import torch as _t
+    reps = 50
+    ev = [[_t.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(reps)]
+    for _ in range(3):
+        solver._solve_pass(0, rhs_pass, out_pass)
+        solver._solve_pass(1, rhs_pass, out_pass)
+    _t.cuda.synchronize()
+    for e in ev:
+        e[0].record()
+        solver._solve_pass(0, rhs_pass, out_pass)
+        e[1].record()
+        solver._solve_pass(1, rhs_pass, out_pass)
+        e[2].record()
+    _t.cuda.synchronize()
+    pass_ms = [float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
+               float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))]
     dom = int(np.argmax(pass_ms))
     achieved = pb / (pass_ms[dom] * 1e-3) / 1e9
     log(f"passes: forward {pass_ms[0] * 1000:.1f} us, backward {pass_ms[1] * 1000:.1f} us, "
@@ -239,6 +257,87 @@ def main():
             "bytes_per_launch": pb,
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def bench_batch(args):
+    """256 independent grid systems (n log-uniform in [1k, 50k], k_i uniform
+    in {1..128}), FP32: one launch per pass for the whole batch, the same
+    replayed from a captured hipGraph, and per-system launches (one solve call
+    per system on its own solver). value = sum(k_i) * steps / time."""
+    import torch
+
+    from cholespy_amd import BatchedSolverF, CholeskySolverF, MatrixType, workloads
+    from oracle import oracle
+
+    t0 = time.time()
+    systems = workloads.batch_systems(count=args.batch_count, seed=0)
+    ks = [k for *_, k in systems]
+    batch = BatchedSolverF([(n, idx[0], idx[1], v, MatrixType.COO) for n, v, idx, _ in systems])
+    log(f"batch setup: {len(systems)} systems, n total {sum(batch.sizes)}, "
+        f"sum k {sum(ks)}, {time.time() - t0:.1f}s; plan {batch._plan_info()}")
+    rng = np.random.default_rng(1)
+    bs = [rng.uniform(-1, 1, (n, k)).astype(np.float32) for (n, *_), k in zip(systems, ks)]
+    b = torch.tensor(np.concatenate([x.ravel() for x in bs]), device="cuda")
+    x = torch.zeros_like(b)
+    run = lambda: batch.solve(ks, b, x)
+    run()
+    torch.cuda.synchronize()
+    batch._check()
+    # accuracy: a few systems against the oracle on their factors
+    xs = x.cpu().numpy()
+    offs = np.concatenate([[0], np.cumsum([n * k for (n, *_), k in zip(systems, ks)])])
+    worst = 0.0
+    for i in range(0, len(systems), max(1, len(systems) // 8)):
+        n = systems[i][0]
+        ref = oracle.solve(batch._factor(i), bs[i], np.float32)
+        got = xs[offs[i]:offs[i + 1]].reshape(n, ks[i])
+        worst = max(worst, float(np.abs(got - ref).max() / np.abs(ref).max()))
+    ms_batch = time_events(run, args.steps)
+    # hipGraph: capture one batched solve on torch's capture stream, replay.
+    ms_graph = None
+    try:
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            batch.solve(ks, b, x, torch.cuda.current_stream().cuda_stream)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g):
+            batch.solve(ks, b, x, torch.cuda.current_stream().cuda_stream)
+        ms_graph = time_events(g.replay, args.steps)
+        batch._check()
+    except Exception as e:  # pragma: no cover - reported, not fatal
+        log(f"graph capture failed: {e}")
+    # per-system launches on separate solvers (same factors: same input)
+    singles = []
+    for n, v, idx, _ in systems:
+        singles.append(CholeskySolverF(n, torch.tensor(idx[0], device="cuda"), torch.tensor(idx[1], device="cuda"),
+                                       torch.tensor(v, device="cuda"), MatrixType.COO))
+    views_b = [b[offs[i]:offs[i + 1]].view(systems[i][0], ks[i]) for i in range(len(systems))]
+    views_x = [x[offs[i]:offs[i + 1]].view(systems[i][0], ks[i]) for i in range(len(systems))]
+
+    def run_singles():
+        for sv, bv, xv in zip(singles, views_b, views_x):
+            sv.solve(bv, xv)
+    ms_singles = time_events(run_singles, max(3, args.steps // 20))
+    best = min(t for t in (ms_batch, ms_graph) if t is not None)
+    cols = sum(ks)
+    log(f"batched {ms_batch:.3f} ms, graph {ms_graph}, per-system {ms_singles:.3f} ms, worst rel err {worst:.2e}")
+    out = {
+        "metric": METRIC,
+        "value": cols * 1000.0 / best,
+        "unit": "RHS-cols/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": best,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {len(systems)} grid systems, n in [1k,50k], k_i in 1..128",
+                   "n_total": int(sum(batch.sizes)), "sum_k": cols, "ms_batched": ms_batch,
+                   "ms_graph": ms_graph, "ms_per_system": ms_singles,
+                   "batched_vs_per_system": ms_singles / best, "rel_err_f32_vs_oracle": worst,
+                   "plan": batch._plan_info()},
     }
     print(json.dumps(out))
 

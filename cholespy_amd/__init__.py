@@ -27,6 +27,9 @@ except ImportError as exc:  # pragma: no cover - exercised only on a broken buil
 MatrixType = _core.MatrixType
 CholeskySolverF = _core.CholeskySolverF
 CholeskySolverD = _core.CholeskySolverD
+BatchedSolverF = _core.BatchedSolverF
+BatchedSolverD = _core.BatchedSolverD
 __version__ = _core.__version__
 
-__all__ = ["MatrixType", "CholeskySolverF", "CholeskySolverD", "__version__"]
+__all__ = ["MatrixType", "CholeskySolverF", "CholeskySolverD", "BatchedSolverF", "BatchedSolverD",
+           "__version__"]

@@ -31,7 +31,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare"]
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare", "-pthread"]
 
 DEVICE_SOURCES = ["analysis.hip", "trsv_rows.hip", "trsv_blocks.hip", "refine.hip", "grad.hip", "capi.hip"]
 PLAN_SOURCES = ["block_plan.cpp"]   # host code linked into libcholespy_amd.so
@@ -85,7 +85,7 @@ def build(verbose: bool = False) -> dict:
     ext = PKG / EXT_NAME
     host_objs = [str(OBJ / (Path(s).stem + ".o")) for s in HOST_SOURCES]
     _run([CXX, "-shared", *host_objs, "-o", str(ext), f"-L{PKG}", "-lcholespy_amd",
-          "-Wl,-rpath,$ORIGIN"])
+          "-Wl,-rpath,$ORIGIN", "-pthread"])
     return {"lib": str(lib), "ext": str(ext)}
 
 

@@ -48,6 +48,9 @@ __global__ __launch_bounds__(kRowBlock) void level_analysis(int n, const int *__
         }
         next = __shfl(pre, 0, kWave);
     }
+    // One finish per workgroup, after all its wavefronts are done.
+    __syncthreads();
+    if (threadIdx.x == 0) finish_queues(qa);
 }
 
 const void *level_analysis_kernel(bool lower) {
@@ -58,7 +61,7 @@ const void *level_analysis_kernel(bool lower) {
 hipError_t launch_level_analysis(bool lower, int n, const int *rowptr, const int *cols, int *lvl1,
                                  int *status, int grid, QueueState &queues, hipStream_t stream) {
     QueueArgs qa;
-    queues.next_launch(qa, grid * (kRowBlock / kWave), (n + kRowChunk - 1) / kRowChunk);
+    queues.fill(qa);
     if (lower)
         level_analysis<true><<<grid, kRowBlock, 0, stream>>>(n, rowptr, cols, lvl1, status, qa);
     else
@@ -66,15 +69,9 @@ hipError_t launch_level_analysis(bool lower, int n, const int *rowptr, const int
     return hipGetLastError();
 }
 
-void QueueState::next_launch(QueueArgs &qa, int workers, int tickets) {
+void QueueState::fill(QueueArgs &qa) const {
     qa.ctr = ctr;
     qa.nq = nq;
-    for (int q = 0; q < kMaxQueues; ++q) qa.base[q] = base[q];
-    for (int q = 0; q < nq; ++q) {
-        const long long own = q < tickets ? (tickets - q + nq - 1) / nq : 0;
-        const long long wk = q < workers ? (workers - q + nq - 1) / nq : 0;
-        base[q] += (unsigned long long)(own + wk);   // each worker takes its tickets plus one miss
-    }
 }
 
 int max_coresident_blocks(const void *kernel, int block, size_t lds_bytes) {

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/cholespy_amd.h"
@@ -88,6 +89,12 @@ struct DeviceBlockPlan {
 
 }  // namespace
 
+struct cholespy_gpu_solver;
+namespace {
+BlockPlanOptions plan_options();
+int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P);
+}  // namespace
+
 struct cholespy_gpu_solver {
     int n = 0;
     int fbytes = 4;
@@ -124,11 +131,76 @@ struct cholespy_gpu_solver {
     int grid_blocks[2] = {0, 0};              // [forward]
     // Ticket queues per kernel: [0..1] analysis upper/lower, [2..3] row pass
     // backward/forward, [4..5] block pass backward/forward.
-    unsigned long long *qctr = nullptr;
+    unsigned int *qctr = nullptr;
     QueueState queues[6];
     unsigned long long *trace[2] = {nullptr, nullptr};   // block pass timelines [backward, forward]
+    // Batches of independent systems (cholespy_batch_*): rows are
+    // concatenated; per system its size, partial-slot count and host factor
+    // until finalize; per call the RHS counts and vector bases.
+    int nsys = 1;
+    bool is_batch = false;
+    std::vector<int> sys_n, sys_row0, sys_slots, sys_slot0;
+    struct HostFactor {
+        std::vector<int> perm, colptr, rowidx;
+        std::vector<double> val;
+    };
+    std::vector<HostFactor> pending;
+    int *blk_sys = nullptr, *sys_k = nullptr;
+    long long *sys_vbase = nullptr, *sys_pbase = nullptr;
+    std::vector<int> last_k;   // RHS counts the per-system arrays hold
     std::vector<int> host_levels[2], host_row_level[2];
 };
+
+namespace {
+// Block-size limits; CHOLESPY_MAX_BLOCK and CHOLESPY_LEAF_MAX override
+// them (tuning sweeps).
+BlockPlanOptions plan_options() {
+    BlockPlanOptions opt;
+    if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
+    return opt;
+}
+
+// Copies a (possibly merged) block plan to the device in the handle's precision.
+int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
+    DeviceBlockPlan &d = s->plan;
+    d.release();
+    d.nblocks = P.nblocks;
+    d.nslots = P.rptr[P.nblocks];
+    d.max_rows = P.max_rows;
+    d.n_fwd_tasks = (int)(P.fwd_task.size() / 3);
+    d.n_bwd_tasks = (int)(P.bwd_task.size() / 3);
+    d.levels_fwd = d.levels_bwd = 0;
+    for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
+    for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
+    d.m_entries = P.moff[P.nblocks];
+    HIP_TRY(upload(&d.start, P.start.data(), P.start.size()));
+    HIP_TRY(upload(&d.rptr, P.rptr.data(), P.rptr.size()));
+    HIP_TRY(upload(&d.rlist, P.rlist.data(), P.rlist.size()));
+    HIP_TRY(upload(&d.moff, P.moff.data(), P.moff.size()));
+    HIP_TRY(upload(&d.cptr, P.contrib_ptr.data(), P.contrib_ptr.size()));
+    HIP_TRY(upload(&d.cslot, P.contrib_slot.data(), P.contrib_slot.size()));
+    HIP_TRY(upload(&d.ftask, P.fwd_task.data(), P.fwd_task.size()));
+    HIP_TRY(upload(&d.btask, P.bwd_task.data(), P.bwd_task.size()));
+    const size_t me = (size_t)d.m_entries;
+    if (s->fbytes == 4) {
+        std::vector<float> f(me), b(me);
+        for (size_t i = 0; i < me; ++i) {
+            f[i] = (float)P.mfwd[i];
+            b[i] = (float)P.mbwd[i];
+        }
+        HIP_TRY(upload(reinterpret_cast<float **>(&d.mfwd), f.data(), me));
+        HIP_TRY(upload(reinterpret_cast<float **>(&d.mbwd), b.data(), me));
+    } else {
+        HIP_TRY(upload(reinterpret_cast<double **>(&d.mfwd), P.mfwd.data(), me));
+        HIP_TRY(upload(reinterpret_cast<double **>(&d.mbwd), P.mbwd.data(), me));
+    }
+    s->grid_blocks[0] = s->grid_blocks[1] = 0;
+    s->buffers_path = -1;
+    return CHOLESPY_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -183,11 +255,11 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
     if (e == hipSuccess) e = hipMemset(s->status, 0, 2 * sizeof(int));
     if (e == hipSuccess) e = dev_alloc(&s->perm, (size_t)n);
     if (e == hipSuccess) e = dev_alloc(&s->lvl1, (size_t)n);
-    const size_t qwords = 6 * (size_t)kMaxQueues * kQueueStride;
+    const size_t qwords = 6 * kQueueWords;
     if (e == hipSuccess) e = dev_alloc(&s->qctr, qwords);
-    if (e == hipSuccess) e = hipMemset(s->qctr, 0, qwords * sizeof(unsigned long long));
+    if (e == hipSuccess) e = hipMemset(s->qctr, 0, qwords * sizeof(unsigned int));
     if (e == hipSuccess)
-        for (int i = 0; i < 6; ++i) s->queues[i].ctr = s->qctr + (size_t)i * kMaxQueues * kQueueStride;
+        for (int i = 0; i < 6; ++i) s->queues[i].ctr = s->qctr + (size_t)i * kQueueWords;
     if (e != hipSuccess) {
         cholespy_solver_destroy(s);
         return hip_fail(e, "cholespy_solver_create");
@@ -298,44 +370,10 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     s->uploaded = true;
     s->analysed[0] = s->analysed[1] = false;
 
-    // Block plan for the partitioned-inverse path. CHOLESPY_MAX_BLOCK and
-    // CHOLESPY_LEAF_MAX override the block-size limits (tuning sweeps).
-    BlockPlanOptions opt;
-    if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
-    if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
+    BlockPlanOptions opt = plan_options();
     BlockPlan P = build_block_plan(n, colptr, rowidx, val, opt);
-    DeviceBlockPlan &d = s->plan;
-    d.release();
-    d.nblocks = P.nblocks;
-    d.nslots = P.rptr[P.nblocks];
-    d.max_rows = P.max_rows;
-    d.n_fwd_tasks = (int)(P.fwd_task.size() / 3);
-    d.n_bwd_tasks = (int)(P.bwd_task.size() / 3);
-    d.levels_fwd = d.levels_bwd = 0;
-    for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
-    for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
-    d.m_entries = P.moff[P.nblocks];
-    HIP_TRY(upload(&d.start, P.start.data(), P.start.size()));
-    HIP_TRY(upload(&d.rptr, P.rptr.data(), P.rptr.size()));
-    HIP_TRY(upload(&d.rlist, P.rlist.data(), P.rlist.size()));
-    HIP_TRY(upload(&d.moff, P.moff.data(), P.moff.size()));
-    HIP_TRY(upload(&d.cptr, P.contrib_ptr.data(), P.contrib_ptr.size()));
-    HIP_TRY(upload(&d.cslot, P.contrib_slot.data(), P.contrib_slot.size()));
-    HIP_TRY(upload(&d.ftask, P.fwd_task.data(), P.fwd_task.size()));
-    HIP_TRY(upload(&d.btask, P.bwd_task.data(), P.bwd_task.size()));
-    const size_t me = (size_t)d.m_entries;
-    if (s->fbytes == 4) {
-        std::vector<float> f(me), b(me);
-        for (size_t i = 0; i < me; ++i) {
-            f[i] = (float)P.mfwd[i];
-            b[i] = (float)P.mbwd[i];
-        }
-        HIP_TRY(upload(reinterpret_cast<float **>(&d.mfwd), f.data(), me));
-        HIP_TRY(upload(reinterpret_cast<float **>(&d.mbwd), b.data(), me));
-    } else {
-        HIP_TRY(upload(reinterpret_cast<double **>(&d.mfwd), P.mfwd.data(), me));
-        HIP_TRY(upload(reinterpret_cast<double **>(&d.mbwd), P.mbwd.data(), me));
-    }
+    int rc = upload_plan(s, P);
+    if (rc != CHOLESPY_OK) return rc;
     s->buffers_path = -1;
     return CHOLESPY_OK;
 }
@@ -408,9 +446,7 @@ int cholespy_solver_check(cholespy_gpu_solver *s) {
     HIP_TRY(hipMemcpy(status, s->status, sizeof(status), hipMemcpyDeviceToHost));
     if (status[0] != 0 || status[1] != 0) {
         HIP_TRY(hipMemset(s->status, 0, 2 * sizeof(int)));
-        HIP_TRY(hipMemset(s->qctr, 0, 6 * (size_t)kMaxQueues * kQueueStride * sizeof(unsigned long long)));
-        for (auto &q : s->queues)
-            for (auto &b : q.base) b = 0;
+        HIP_TRY(hipMemset(s->qctr, 0, 6 * kQueueWords * sizeof(unsigned int)));
         // Work buffers may hold partial results: restore the sentinel state.
         if (s->work_elems) {
             HIP_TRY(launch_fill_sentinel(s->y, s->work_elems, s->fbytes, nullptr));
@@ -509,6 +545,12 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.part = s->part;
     bf.status = s->status;
     bf.xmode = xmode;
+    if (s->is_batch) {
+        bf.blk_sys = s->blk_sys;
+        bf.sys_k = s->sys_k;
+        bf.sys_vbase = s->sys_vbase;
+        bf.sys_pbase = s->sys_pbase;
+    }
     for (int pass = 0; pass < 2; ++pass) {
         if (only_pass >= 0 && pass != only_pass) continue;
         const bool forward = pass == 0;
@@ -531,6 +573,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
 int solve_impl(cholespy_gpu_solver *s, int fbytes, int k, const void *b, void *x, void *stream,
                int only_pass = -1) {
     if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    if (s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "batch handles solve with cholespy_batch_solve_*");
     if (s->fbytes != fbytes)
         return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "solver precision does not match the call");
     if (k > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
@@ -675,6 +718,219 @@ int cholespy_solver_get_trace(const cholespy_gpu_solver *s, int forward, uint64_
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(out, t, (size_t)std::min(have, words) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return CHOLESPY_OK;
+}
+
+int cholespy_batch_create(int count, const int *n, int float_bytes, cholespy_gpu_solver **out) {
+    if (!out || count < 1 || !n) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "bad batch arguments");
+    int64_t total = 0;
+    for (int i = 0; i < count; ++i) {
+        if (n[i] < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative system size");
+        total += n[i];
+    }
+    if (total > 0x7fffffff) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "batch too large for 32-bit rows");
+    int rc = cholespy_solver_create((int)total, float_bytes, out);
+    if (rc != CHOLESPY_OK) return rc;
+    cholespy_gpu_solver *s = *out;
+    s->is_batch = true;
+    s->nsys = count;
+    s->sys_n.assign(n, n + count);
+    s->sys_row0.assign((size_t)count, 0);
+    for (int i = 1; i < count; ++i) s->sys_row0[i] = s->sys_row0[i - 1] + n[i - 1];
+    s->pending.assign((size_t)count, {});
+    return CHOLESPY_OK;
+}
+
+int cholespy_batch_set_factor(cholespy_gpu_solver *s, int index, const int *perm, const int *colptr,
+                              const int *rowidx, const double *val, int64_t nnz) {
+    if (!s || !s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "not a batch handle");
+    if (index < 0 || index >= s->nsys) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "system index out of range");
+    const int n = s->sys_n[index];
+    if (!perm || !colptr || !rowidx || !val) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    if (nnz != (int64_t)colptr[n] || colptr[0] != 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "colptr does not match nnz");
+    std::vector<char> seen((size_t)n, 0);
+    for (int i = 0; i < n; ++i) {
+        if (perm[i] < 0 || perm[i] >= n || seen[perm[i]])
+            return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "perm is not a permutation");
+        seen[perm[i]] = 1;
+    }
+    for (int j = 0; j < n; ++j) {
+        if (colptr[j + 1] <= colptr[j] || rowidx[colptr[j]] != j)
+            return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "column without leading diagonal");
+        for (int p = colptr[j] + 1; p < colptr[j + 1]; ++p)
+            if (rowidx[p] <= rowidx[p - 1] || rowidx[p] >= n)
+                return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "row indices not ascending in range");
+    }
+    auto &h = s->pending[index];
+    h.perm.assign(perm, perm + n);
+    h.colptr.assign(colptr, colptr + n + 1);
+    h.rowidx.assign(rowidx, rowidx + nnz);
+    h.val.assign(val, val + nnz);
+    return CHOLESPY_OK;
+}
+
+int cholespy_batch_finalize(cholespy_gpu_solver *s) {
+    if (!s || !s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "not a batch handle");
+    const int ns = s->nsys;
+    for (int i = 0; i < ns; ++i)
+        if ((int)s->pending[i].colptr.size() != s->sys_n[i] + 1)
+            return fail(CHOLESPY_ERR_STATE, "a system of the batch has no factor");
+    // Per-system plans, built on host threads.
+    const BlockPlanOptions opt = plan_options();
+    std::vector<BlockPlan> plans((size_t)ns);
+    {
+        const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned w = 0; w < hw; ++w)
+            pool.emplace_back([&, w]() {
+                for (int i = (int)w; i < ns; i += (int)hw) {
+                    const auto &h = s->pending[i];
+                    plans[i] = build_block_plan(s->sys_n[i], h.colptr.data(), h.rowidx.data(), h.val.data(), opt);
+                }
+            });
+        for (auto &t : pool) t.join();
+    }
+    // Merge: blocks, slots, contributions and dense values concatenated with
+    // offsets; tasks of all systems interleaved by block level.
+    BlockPlan G;
+    G.n = s->n;
+    std::vector<int> blk_sys, gperm((size_t)s->n);
+    s->sys_slots.assign((size_t)ns, 0);
+    s->sys_slot0.assign((size_t)ns, 0);
+    int B0 = 0, S0 = 0, C0 = 0;
+    int64_t M0 = 0;
+    struct TaskRef { int level, sys, idx; };
+    std::vector<TaskRef> ftasks, btasks;
+    for (int i = 0; i < ns; ++i) {
+        const BlockPlan &P = plans[i];
+        const int R0 = s->sys_row0[i];
+        s->sys_slot0[i] = S0;
+        s->sys_slots[i] = P.rptr[P.nblocks];
+        for (int r = 0; r < s->sys_n[i]; ++r) gperm[(size_t)R0 + r] = R0 + s->pending[i].perm[r];
+        for (int b = 0; b < P.nblocks; ++b) {
+            G.start.push_back(P.start[b] + R0);
+            G.rptr.push_back(P.rptr[b] + S0);
+            G.moff.push_back(P.moff[b] + M0);
+            G.level_fwd.push_back(P.level_fwd[b]);
+            G.level_bwd.push_back(P.level_bwd[b]);
+            blk_sys.push_back(i);
+        }
+        for (int v : P.rlist) G.rlist.push_back(v + R0);
+        G.mfwd.insert(G.mfwd.end(), P.mfwd.begin(), P.mfwd.end());
+        G.mbwd.insert(G.mbwd.end(), P.mbwd.begin(), P.mbwd.end());
+        for (int r = 0; r < s->sys_n[i]; ++r) G.contrib_ptr.push_back(P.contrib_ptr[r] + C0);
+        for (int v : P.contrib_slot) G.contrib_slot.push_back(v + S0);
+        for (size_t t = 0; t < P.fwd_task.size() / 3; ++t)
+            ftasks.push_back({P.level_fwd[P.fwd_task[3 * t]], i, (int)t});
+        for (size_t t = 0; t < P.bwd_task.size() / 3; ++t)
+            btasks.push_back({P.level_bwd[P.bwd_task[3 * t]], i, (int)t});
+        G.max_m = std::max(G.max_m, P.max_m);
+        G.max_rows = std::max(G.max_rows, P.max_rows);
+        G.nnz_l += P.nnz_l;
+        s->sys_row0[i] = R0;
+        B0 += P.nblocks;
+        S0 += P.rptr[P.nblocks];
+        C0 += P.contrib_ptr[s->sys_n[i]];
+        M0 += P.moff[P.nblocks];
+    }
+    G.nblocks = B0;
+    G.start.push_back(s->n);
+    G.rptr.push_back(S0);
+    G.moff.push_back(M0);
+    G.contrib_ptr.push_back(C0);
+    std::vector<int> blk0((size_t)ns + 1, 0);
+    for (int i = 0; i < ns; ++i) blk0[i + 1] = blk0[i] + plans[i].nblocks;
+    auto emit = [&](std::vector<TaskRef> &refs, bool fwd, std::vector<int> &out) {
+        std::stable_sort(refs.begin(), refs.end(), [](const TaskRef &x, const TaskRef &y) { return x.level < y.level; });
+        for (const TaskRef &r : refs) {
+            const std::vector<int> &src = fwd ? plans[r.sys].fwd_task : plans[r.sys].bwd_task;
+            out.push_back(src[3 * r.idx] + blk0[r.sys]);
+            out.push_back(src[3 * r.idx + 1]);
+            out.push_back(src[3 * r.idx + 2]);
+        }
+    };
+    emit(ftasks, true, G.fwd_task);
+    emit(btasks, false, G.bwd_task);
+    plans.clear();
+    HIP_TRY(hipMemcpy(s->perm, gperm.data(), (size_t)s->n * sizeof(int), hipMemcpyHostToDevice));
+    int rc = upload_plan(s, G);
+    if (rc != CHOLESPY_OK) return rc;
+    dev_free(s->blk_sys);
+    HIP_TRY(upload(&s->blk_sys, blk_sys.data(), blk_sys.size()));
+    dev_free(s->sys_k);
+    dev_free(s->sys_vbase);
+    dev_free(s->sys_pbase);
+    HIP_TRY(dev_alloc(&s->sys_k, (size_t)ns));
+    HIP_TRY(dev_alloc(&s->sys_vbase, (size_t)ns));
+    HIP_TRY(dev_alloc(&s->sys_pbase, (size_t)ns));
+    s->last_k.clear();
+    s->pending.clear();
+    s->uploaded = true;
+    s->analysed[0] = s->analysed[1] = true;   // the block path needs no level analysis
+    s->nnz = G.nnz_l;
+    return CHOLESPY_OK;
+}
+
+namespace {
+
+int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void *b, void *x, void *stream) {
+    if (!s || !s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "not a batch handle");
+    if (s->fbytes != fbytes) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "batch precision does not match the call");
+    if (!n_rhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null n_rhs");
+    if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "batch not finalised");
+    const int ns = s->nsys;
+    std::vector<int> ks(n_rhs, n_rhs + ns);
+    size_t elems = 0, pe = 0;
+    for (int i = 0; i < ns; ++i) {
+        if (ks[i] < 0 || ks[i] > kMaxRhs)
+            return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
+        elems += (size_t)s->sys_n[i] * ks[i];
+        pe += (size_t)s->sys_slots[i] * ks[i];
+    }
+    if (elems == 0) return CHOLESPY_OK;
+    if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    if (ks != s->last_k) {
+        std::vector<long long> vb((size_t)ns), pb((size_t)ns);
+        long long voff = 0, poff = 0;
+        for (int i = 0; i < ns; ++i) {
+            vb[i] = voff - (long long)s->sys_row0[i] * ks[i];
+            pb[i] = poff - (long long)s->sys_slot0[i] * ks[i];
+            voff += (long long)s->sys_n[i] * ks[i];
+            poff += (long long)s->sys_slots[i] * ks[i];
+        }
+        HIP_TRY(hipMemcpy(s->sys_k, ks.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->sys_vbase, vb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->sys_pbase, pb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
+        if (elems > s->work_elems) {
+            dev_free(s->y);
+            dev_free(s->z);
+            HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), elems * (size_t)fbytes));
+            HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), elems * (size_t)fbytes));
+            s->work_elems = elems;
+        }
+        if (pe > s->part_elems) {
+            dev_free(s->part);
+            HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->part), pe * (size_t)fbytes));
+            s->part_elems = pe;
+        }
+        HIP_TRY(launch_fill_sentinel(s->y, elems, fbytes, st));
+        HIP_TRY(launch_fill_sentinel(s->z, elems, fbytes, st));
+        if (pe) HIP_TRY(launch_fill_sentinel(s->part, pe, fbytes, st));
+        s->last_k = ks;
+        s->buffers_path = CHOLESPY_PATH_BLOCKS;
+    }
+    return solve_blocks(s, 0, b, x, st);
+}
+
+}  // namespace
+
+int cholespy_batch_solve_f32(cholespy_gpu_solver *s, const int *n_rhs, const float *b, float *x, void *stream) {
+    return batch_solve(s, 4, n_rhs, b, x, stream);
+}
+
+int cholespy_batch_solve_f64(cholespy_gpu_solver *s, const int *n_rhs, const double *b, double *x,
+                             void *stream) {
+    return batch_solve(s, 8, n_rhs, b, x, stream);
 }
 
 int cholespy_solver_set_path(cholespy_gpu_solver *s, int path) {

@@ -127,24 +127,35 @@ __device__ __forceinline__ int wait_nonzero(const int *p, int *status) {
 // MI355X). To avoid serialising on one atomic address, tickets are spread
 // over nq queues: ticket t belongs to queue t mod nq, worker w serves queue
 // w mod nq (the first nq workers are dispatched first, so every queue has a
-// running worker). Counters only grow: each launch passes the value every
-// counter had at its start (`base`), so no reset is needed between launches.
+// running worker). Each launch starts from zeroed counters and leaves them
+// zeroed: the last workgroup to finish resets them, so launches need no
+// host-side state and can be replayed from a captured graph.
 constexpr int kMaxQueues = 16;
 constexpr int kQueueStride = 16;   // 128 bytes between counters
 
 struct QueueArgs {
-    unsigned long long *ctr;
-    unsigned long long base[kMaxQueues];
+    unsigned int *ctr;    // nq counters kQueueStride apart, then the done counter
     int nq;
 };
 
 // Next ticket of queue q, or -1 when the queue is exhausted (n tickets in all).
 __device__ __forceinline__ int take_ticket(const QueueArgs &qa, int q, int n) {
-    const unsigned long long v =
-        __hip_atomic_fetch_add(qa.ctr + (size_t)q * kQueueStride, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const long long j = (long long)(v - qa.base[q]);
-    const long long t = j * qa.nq + q;
+    const unsigned int j =
+        __hip_atomic_fetch_add(qa.ctr + (size_t)q * kQueueStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long t = (long long)j * qa.nq + q;
     return t < n ? (int)t : -1;
+}
+
+// Called once per workgroup by one thread after its last ticket: the last
+// workgroup of the grid zeroes the counters for the next launch.
+__device__ __forceinline__ void finish_queues(const QueueArgs &qa) {
+    unsigned int *done = qa.ctr + (size_t)kMaxQueues * kQueueStride;
+    const unsigned int d = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (d + 1 == gridDim.x * gridDim.y * gridDim.z) {
+        for (int q = 0; q < qa.nq; ++q)
+            __hip_atomic_store(qa.ctr + (size_t)q * kQueueStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 __device__ __forceinline__ int wave_max(int v) {

@@ -8,15 +8,14 @@ namespace cholespy_amd {
 
 struct QueueArgs;   // common.hpp
 
-// Host-side bookkeeping of one kernel's ticket queues (see common.hpp).
+// Device counters of one kernel's ticket queues (see common.hpp): kMaxQueues
+// counters 16 words apart, then a done counter; zero between launches.
 struct QueueState {
-    unsigned long long *ctr = nullptr;   // device, kMaxQueues * 16 counters
-    unsigned long long base[16] = {0};
+    unsigned int *ctr = nullptr;
     int nq = 16;
-    // Fills the launch arguments and advances the bases past this launch, in
-    // which `workers` workers take `tickets` tickets in all.
-    void next_launch(QueueArgs &qa, int workers, int tickets);
+    void fill(QueueArgs &qa) const;
 };
+constexpr size_t kQueueWords = 16 * 16 + 16;   // counters + done word (padded)
 
 // Threads per workgroup of the persistent row kernels (4 wavefronts).
 constexpr int kRowBlock = 256;
@@ -68,6 +67,9 @@ struct BlockPassBuffers {
     // 2 = x (double) += z (refinement update).
     int xmode = 0;
     unsigned long long *trace = nullptr;   // optional task timeline (4 words per task)
+    // Batches of independent systems (nullptr for a single system).
+    const int *blk_sys = nullptr, *sys_k = nullptr;
+    const long long *sys_vbase = nullptr, *sys_pbase = nullptr;
 };
 size_t block_pass_lds(int float_bytes, int max_rows);
 const void *block_pass_kernel(int float_bytes, bool forward);

@@ -57,11 +57,32 @@ struct BlockPassArgs {
     void *x;
     int *status;
     int xmode;
+    // Several independent systems in one launch: block -> system, and per
+    // system its right-hand-side count and the (possibly negative) bases
+    // that place row r / slot s at base + r * k + t in the packed vectors.
+    // nullptr = one system with a.k columns and zero bases.
+    const int *blk_sys;
+    const int *sys_k;
+    const long long *sys_vbase;
+    const long long *sys_pbase;
     QueueArgs qa;
     // Optional task timeline (nullptr = off): per task, 4 words =
     // {start, after the gather phase, end, workgroup id} in wall-clock ticks (100 MHz).
     unsigned long long *trace;
 };
+
+// Right-hand-side layout of one task's system.
+struct VecMap {
+    int k;
+    long long vbase, pbase;
+};
+
+template <typename F>
+__device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, int blk) {
+    if (!a.blk_sys) return VecMap{a.k, 0, 0};
+    const int sys = a.blk_sys[blk];
+    return VecMap{a.sys_k[sys], a.sys_vbase[sys], a.sys_pbase[sys]};
+}
 
 // Ticket handling per workgroup: thread 0 takes tickets, the LDS word s_next
 // broadcasts them at task boundaries (after a barrier).
@@ -99,8 +120,12 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const int R = a.rptr[blk + 1] - a.rptr[blk];
         const int rows = m + R;
         const F *M = a.M + a.moff[blk];
-        for (int t0 = 0; t0 < a.k; t0 += KT) {
-            const int kt = min(KT, a.k - t0);
+        const VecMap vm = vec_map(a, blk);
+        const int k = vm.k;
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+        const F *bv = a.b + vm.vbase;
+        for (int t0 = 0; t0 < k; t0 += KT) {
+            const int kt = min(KT, k - t0);
             // r_B: each thread owns up to kPairs (row, column) pairs and
             // walks their partial-slot lists together, 4 slots per pair per
             // round, so up to 4 * kPairs slot loads are in flight at once.
@@ -116,7 +141,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     v[q] = F(0);
                     if (idx < m * KT && t < kt) {
                         const int row = a0 + i;
-                        v[q] = a.b[(size_t)a.perm[row] * a.k + col[q]];
+                        v[q] = bv[(long long)a.perm[row] * k + col[q]];
                         p0[q] = a.cptr[row];
                         p1[q] = a.cptr[row + 1];
                     }
@@ -131,7 +156,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                         for (int j = 0; j < 4; ++j) {
                             const int p = p0[q] + r + j;
                             sl[q][j] = p < p1[q] ? a.cslot[p] : -1;
-                            u[q][j] = sl[q][j] >= 0 ? load_bits(&a.part[(size_t)sl[q][j] * a.k + col[q]]) : U(0);
+                            u[q][j] = sl[q][j] >= 0 ? load_bits(&pv[(long long)sl[q][j] * k + col[q]]) : U(0);
                             any |= sl[q][j] >= 0;
                         }
                     if (!any) break;
@@ -141,7 +166,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                         for (int j = 0; j < 4; ++j) {
                             F x = from_bits<F>(u[q][j]);
                             if (sl[q][j] >= 0 && u[q][j] == Bits<F>::sentinel)
-                                x = wait_value(&a.part[(size_t)sl[q][j] * a.k + col[q]], a.status);
+                                x = wait_value(&pv[(long long)sl[q][j] * k + col[q]], a.status);
                             v[q] += x;
                         }
                 }
@@ -197,11 +222,11 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     for (int g = 1; g < kSegs; ++g) s += red[(g * kLanes + lr) * KT + t];
                     const int col = t0 + t;
                     if (rho < m) {
-                        const size_t o = (size_t)(a0 + rho) * a.k + col;
-                        a.y[o] = s;
-                        a.z[o] = from_bits<F>(Bits<F>::sentinel);
+                        const long long o = (long long)(a0 + rho) * k + col;
+                        yv[o] = s;
+                        zv[o] = from_bits<F>(Bits<F>::sentinel);
                     } else {
-                        publish(&a.part[(size_t)(a.rptr[blk] + rho - m) * a.k + col], s);
+                        publish(&pv[(long long)(a.rptr[blk] + rho - m) * k + col], s);
                     }
                 }
             }
@@ -216,6 +241,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         tk = s_next;
         __syncthreads();
     }
+    if (tid == 0) finish_queues(a.qa);
 }
 
 template <typename F, int KT>
@@ -240,12 +266,15 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         const int R = a.rptr[blk + 1] - r0;
         const int rows = m + R;
         const F *M = a.M + a.moff[blk];
-        for (int t0 = 0; t0 < a.k; t0 += KT) {
-            const int kt = min(KT, a.k - t0);
+        const VecMap vm = vec_map(a, blk);
+        const int k = vm.k;
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+        for (int t0 = 0; t0 < k; t0 += KT) {
+            const int kt = min(KT, k - t0);
             // v = [y_B ; z_R]: y from the forward pass (complete), z waited on.
             for (int base = tid; base < rows * KT; base += kTaskThreads * kGatherUnroll) {
                 U u[kGatherUnroll];
-                int addr[kGatherUnroll];
+                long long addr[kGatherUnroll];
 #pragma unroll
                 for (int j = 0; j < kGatherUnroll; ++j) {
                     const int idx = base + j * kTaskThreads;
@@ -254,10 +283,10 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     u[j] = U(0);
                     if (idx < rows * KT && t < kt) {
                         if (rho < m) {
-                            u[j] = as_bits(a.y[(size_t)(a0 + rho) * a.k + t0 + t]);
+                            u[j] = as_bits(yv[(long long)(a0 + rho) * k + t0 + t]);
                         } else {
-                            addr[j] = a.rlist[r0 + rho - m] * a.k + t0 + t;
-                            u[j] = load_bits(&a.z[addr[j]]);
+                            addr[j] = (long long)a.rlist[r0 + rho - m] * k + t0 + t;
+                            u[j] = load_bits(&zv[addr[j]]);
                         }
                     }
                 }
@@ -266,7 +295,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     const int idx = base + j * kTaskThreads;
                     if (idx >= rows * KT) break;
                     F v = from_bits<F>(u[j]);
-                    if (addr[j] >= 0 && u[j] == Bits<F>::sentinel) v = wait_value(&a.z[addr[j]], a.status);
+                    if (addr[j] >= 0 && u[j] == Bits<F>::sentinel) v = wait_value(&zv[addr[j]], a.status);
                     vs[idx] = v;
                 }
             }
@@ -313,8 +342,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
 #pragma unroll
                     for (int g = 1; g < kBSegs; ++g) s += red[(g * kBLanes + lr) * KT + t];
                     const int col = t0 + t;
-                    publish(&a.z[(size_t)(a0 + c) * a.k + col], s);
-                    const size_t o = (size_t)a.perm[a0 + c] * a.k + col;
+                    publish(&zv[(long long)(a0 + c) * k + col], s);
+                    const long long o = vm.vbase + (long long)a.perm[a0 + c] * k + col;
                     if (a.xmode == 0) {
                         static_cast<F *>(a.x)[o] = s;
                     } else {
@@ -328,7 +357,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         // Restore the sentinel in this block's partial slots for the next solve.
         if (first == 0) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
-            for (int idx = tid; idx < R * a.k; idx += kTaskThreads) a.part[(size_t)r0 * a.k + idx] = sent;
+            for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
         }
         if (a.trace && tid == 0) {
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
@@ -339,6 +368,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         tk = s_next;
         __syncthreads();
     }
+    if (tid == 0) finish_queues(a.qa);
 }
 
 template <typename F>
@@ -365,7 +395,11 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.status = bf.status;
     a.xmode = bf.xmode;
     a.trace = bf.trace;
-    queues.next_launch(a.qa, grid, bf.ntasks);
+    a.blk_sys = bf.blk_sys;
+    a.sys_k = bf.sys_k;
+    a.sys_vbase = bf.sys_vbase;
+    a.sys_pbase = bf.sys_pbase;
+    queues.fill(a.qa);
     void *args[] = {&a};
     const size_t lds = block_pass_lds(sizeof(F), bf.max_rows);
     return hipLaunchKernel(block_pass_kernel(sizeof(F), forward), dim3(grid), dim3(kTaskThreads), args,

@@ -141,6 +141,8 @@ __global__ __launch_bounds__(kRowBlock) void row_pass(RowPassArgs<F> a) {
       }
       next = __shfl(pre, 0, kWave);
     }
+    __syncthreads();
+    if (threadIdx.x == 0) finish_queues(a.qa);
 }
 
 template <typename F>
@@ -170,7 +172,7 @@ hipError_t launch(bool lower, const RowPassBuffers &bf, int grid, QueueState &qu
     a.z = static_cast<F *>(bf.z);
     a.x = static_cast<F *>(bf.x);
     a.status = bf.status;
-    queues.next_launch(a.qa, grid * (kRowBlock / kWave), (bf.n + kRowChunk - 1) / kRowChunk);
+    queues.fill(a.qa);
     void *args[] = {&a};
     return hipLaunchKernel(pick<F>(lower, bf.k), dim3(grid), dim3(kRowBlock), args, 0, stream);
 }

@@ -12,6 +12,7 @@
 // where the reference accepts CUDA (2).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
 
 #include <cstring>
 #include <memory>
@@ -355,6 +356,74 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def_property_readonly("_is_cpu", [](const Class &self) { return self.is_cpu(); });
 }
 
+template <typename Float>
+void declare_batch(py::module_ &m, const char *name) {
+    using Class = BatchedCholeskySolver<Float>;
+    py::class_<Class>(m, name,
+                      "Independent sparse SPD systems factored on the host and solved together on the GPU "
+                      "(one launch per triangular pass).")
+        .def(py::init([](py::list systems) {
+                 std::vector<SystemInput> in;
+                 for (py::handle item : systems) {
+                     py::tuple t = py::reinterpret_borrow<py::tuple>(item);
+                     if (t.size() != 5)
+                         throw py::type_error("each system is (n_rows, ii, jj, x, type)");
+                     SystemInput si;
+                     si.n = t[0].cast<int>();
+                     py::object ii_o = as_exportable(t[1]), jj_o = as_exportable(t[2]), x_o = as_exportable(t[3]);
+                     DLView ii(ii_o), jj(jj_o), x(x_o);
+                     si.type = t[4].cast<MatrixType>();
+                     si.ii = to_host<int>(ii);
+                     si.jj = to_host<int>(jj);
+                     si.x = to_host<double>(x);
+                     in.push_back(std::move(si));
+                 }
+                 py::gil_scoped_release release;
+                 return std::make_unique<Class>(in);
+             }),
+             py::arg("systems"))
+        .def(
+            "solve",
+            [](Class &self, std::vector<int> n_rhs, py::object b_o, py::object x_o, uintptr_t stream) {
+                DLView b(b_o), x(x_o);
+                const int bits = 8 * (int)sizeof(Float);
+                for (const DLView *v : {&b, &x})
+                    if (v->device() != Dev::GPU || !v->c_contig() || v->dtype().code != dl::kFloat ||
+                        v->dtype().bits != bits)
+                        throw py::type_error("solve(): b and x must be packed C-contiguous GPU arrays of the "
+                                             "batch dtype.");
+                if ((int)n_rhs.size() != self.count()) throw std::invalid_argument("n_rhs needs one entry per system.");
+                int64_t need = 0;
+                for (int i = 0; i < self.count(); ++i) need += (int64_t)self.sizes()[i] * n_rhs[i];
+                if (b.size() != need || x.size() != need)
+                    throw std::invalid_argument("b and x must hold sum(n_i * n_rhs_i) elements.");
+                self.solve(n_rhs, b.data(), x.data(), reinterpret_cast<void *>(stream));
+            },
+            py::arg("n_rhs"), py::arg("b"), py::arg("x"), py::arg("stream") = 0)
+        .def("_factor",
+             [](const Class &self, int i) {
+                 const Factor &f = self.factor(i);
+                 py::dict d;
+                 d["perm"] = py::array_t<int>(f.perm.size(), f.perm.data());
+                 d["colptr"] = py::array_t<int>(f.colptr.size(), f.colptr.data());
+                 d["rowidx"] = py::array_t<int>(f.rowidx.size(), f.rowidx.data());
+                 d["val"] = py::array_t<double>(f.val.size(), f.val.data());
+                 return d;
+             })
+        .def("_check", [](const Class &self) { throw_on_error(cholespy_solver_check(self.gpu())); })
+        .def("_plan_info",
+             [](const Class &self) {
+                 std::vector<int64_t> v(8, 0);
+                 throw_on_error(cholespy_solver_plan_info(self.gpu(), v.data(), (int)v.size()));
+                 const char *names[] = {"blocks", "m_entries", "levels_fwd", "levels_bwd",
+                                        "fwd_tasks", "bwd_tasks", "slots", "max_rows"};
+                 py::dict d;
+                 for (size_t i = 0; i < v.size(); ++i) d[names[i]] = v[i];
+                 return d;
+             })
+        .def_property_readonly("sizes", [](const Class &self) { return self.sizes(); });
+}
+
 }  // namespace
 
 // dL/d(stored entries) on the GPU (grad.hip through the C ABI): rows / cols
@@ -391,6 +460,8 @@ PYBIND11_MODULE(_cholespy_core, m) {
 
     declare_solver<float>(m, "CholeskySolverF", doc_cholesky_f);
     declare_solver<double>(m, "CholeskySolverD", doc_cholesky_d);
+    declare_batch<float>(m, "BatchedSolverF");
+    declare_batch<double>(m, "BatchedSolverD");
 
     // Release library state when the module is unloaded (src/main.cpp:126-129).
     m.add_object("_shutdown_guard", py::capsule(reinterpret_cast<void *>(1), [](void *) {

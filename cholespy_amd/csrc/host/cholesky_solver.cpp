@@ -1,7 +1,10 @@
 #include "cholesky_solver.hpp"
 
+#include <algorithm>
+#include <exception>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 
 namespace cholespy_amd {
@@ -111,5 +114,68 @@ std::vector<int64_t> CholeskySolver<Float>::plan_info() const {
 
 template class CholeskySolver<float>;
 template class CholeskySolver<double>;
+
+template <typename Float>
+BatchedCholeskySolver<Float>::BatchedCholeskySolver(const std::vector<SystemInput> &systems) {
+    const int ns = (int)systems.size();
+    if (ns == 0) throw std::invalid_argument("A batch needs at least one system.");
+    n_.resize(ns);
+    factors_.resize(ns);
+    for (int i = 0; i < ns; ++i) n_[i] = systems[i].n;
+    // Normalise and factor on host threads; the first failure is re-raised.
+    std::vector<std::exception_ptr> errors(ns);
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < hw; ++w)
+        pool.emplace_back([&, w]() {
+            for (int i = (int)w; i < ns; i += (int)hw) {
+                try {
+                    const SystemInput &in = systems[i];
+                    LowerCSC A = normalize_input(in.n, (int64_t)in.x.size(), in.ii.data(), in.jj.data(),
+                                                 in.x.data(), in.type);
+                    factors_[i] = cholesky_factor(A);
+                } catch (...) {
+                    errors[i] = std::current_exception();
+                }
+            }
+        });
+    for (auto &t : pool) t.join();
+    for (auto &e : errors)
+        if (e) std::rethrow_exception(e);
+    throw_on_error(cholespy_batch_create(ns, n_.data(), (int)sizeof(Float), &gpu_));
+    try {
+        for (int i = 0; i < ns; ++i) {
+            const Factor &f = factors_[i];
+            throw_on_error(cholespy_batch_set_factor(gpu_, i, f.perm.data(), f.colptr.data(), f.rowidx.data(),
+                                                     f.val.data(), f.nnz()));
+        }
+        throw_on_error(cholespy_batch_finalize(gpu_));
+    } catch (...) {
+        cholespy_solver_destroy(gpu_);
+        gpu_ = nullptr;
+        throw;
+    }
+}
+
+template <typename Float>
+BatchedCholeskySolver<Float>::~BatchedCholeskySolver() {
+    if (gpu_) cholespy_solver_destroy(gpu_);
+}
+
+template <typename Float>
+void BatchedCholeskySolver<Float>::solve(const std::vector<int> &n_rhs, const void *b, void *x, void *stream) {
+    if ((int)n_rhs.size() != count()) throw std::invalid_argument("n_rhs needs one entry per system.");
+    for (int k : n_rhs)
+        if (k > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
+    if (std::is_same<Float, float>::value)
+        throw_on_error(cholespy_batch_solve_f32(gpu_, n_rhs.data(), static_cast<const float *>(b),
+                                                static_cast<float *>(x), stream));
+    else
+        throw_on_error(cholespy_batch_solve_f64(gpu_, n_rhs.data(), static_cast<const double *>(b),
+                                                static_cast<double *>(x), stream));
+}
+
+template class BatchedCholeskySolver<float>;
+template class BatchedCholeskySolver<double>;
 
 }  // namespace cholespy_amd

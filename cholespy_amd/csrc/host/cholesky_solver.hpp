@@ -63,6 +63,37 @@ private:
     int n_levels_lower_ = 0, n_levels_upper_ = 0;
 };
 
+// One system of a batch, as given to the constructor (host arrays).
+struct SystemInput {
+    int n = 0;
+    MatrixType type = MatrixType::COO;
+    std::vector<int> ii, jj;
+    std::vector<double> x;
+};
+
+// Many independent systems factored on host threads and solved together on
+// the GPU, one launch per pass (cholespy_batch_* in the C ABI).
+template <typename Float>
+class BatchedCholeskySolver {
+public:
+    explicit BatchedCholeskySolver(const std::vector<SystemInput> &systems);
+    ~BatchedCholeskySolver();
+    BatchedCholeskySolver(const BatchedCholeskySolver &) = delete;
+    BatchedCholeskySolver &operator=(const BatchedCholeskySolver &) = delete;
+    // b, x: packed device arrays (system i: row-major n_i x n_rhs[i]);
+    // enqueued on `stream` (nullptr = null stream).
+    void solve(const std::vector<int> &n_rhs, const void *b, void *x, void *stream = nullptr);
+    int count() const { return (int)n_.size(); }
+    const std::vector<int> &sizes() const { return n_; }
+    const Factor &factor(int i) const { return factors_.at(i); }
+    cholespy_gpu_solver *gpu() const { return gpu_; }
+
+private:
+    std::vector<int> n_;
+    std::vector<Factor> factors_;
+    cholespy_gpu_solver *gpu_ = nullptr;
+};
+
 // Raises the C++ exception matching a C ABI status (invalid argument ->
 // std::invalid_argument, anything else -> std::runtime_error).
 void throw_on_error(int status);

@@ -144,3 +144,22 @@ def smoothing_system(level: int = 7, lam: float = 10.0, seed: int = 0):
     rng = np.random.default_rng(seed)
     rhs = pos + rng.normal(0.0, 0.01, size=pos.shape)
     return n, values, idx, rhs
+
+
+def batch_systems(count: int = 256, seed: int = 0, n_min: int = 1000, n_max: int = 50000,
+                  k_max: int = 128):
+    """Independent grid systems for batched solves: n log-uniform in
+    [n_min, n_max], a 2D 5-point grid nx x ny with aspect ratio log-uniform
+    in [1/4, 4], plus shift * I with shift ~ U[1e-2, 1], and a right-hand-side
+    count uniform in {1..k_max}. Returns a list of (n, values, idx, k)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        r = float(np.exp(rng.uniform(np.log(0.25), np.log(4.0))))
+        nx = max(2, int(round(np.sqrt(n * r))))
+        ny = max(2, int(round(n / nx)))
+        shift = float(rng.uniform(1e-2, 1.0))
+        values, idx = grid2d_laplacian(nx, ny, shift)
+        out.append((nx * ny, values, idx, int(rng.integers(1, k_max + 1))))
+    return out

@@ -144,6 +144,24 @@ int cholespy_solver_upload_matrix(cholespy_gpu_solver *solver, const int *colptr
 int cholespy_solver_solve_refined(cholespy_gpu_solver *solver, int n_rhs, const double *b,
                                   double *x, int iterations, void *stream);
 
+/* Batches of independent systems solved by one launch per pass (no
+ * counterpart in the reference, which solves one system per object).
+ * create: `count` systems of sizes n[i]; set_factor: the factor of system
+ * `index` (same form as cholespy_solver_upload_factor); finalize: plans all
+ * systems (host threads), merges the plans and uploads them. solve: system i
+ * has n_rhs[i] right-hand sides (0..128, host array); b and x are packed
+ * device arrays, system i's row-major (n[i] x n_rhs[i]) block starting
+ * after the blocks of systems 0..i-1. Destroy with cholespy_solver_destroy. */
+int cholespy_batch_create(int count, const int *n, int float_bytes, cholespy_gpu_solver **out);
+int cholespy_batch_set_factor(cholespy_gpu_solver *batch, int index, const int *perm,
+                              const int *colptr, const int *rowidx, const double *val,
+                              int64_t nnz);
+int cholespy_batch_finalize(cholespy_gpu_solver *batch);
+int cholespy_batch_solve_f32(cholespy_gpu_solver *batch, const int *n_rhs, const float *b,
+                             float *x, void *stream);
+int cholespy_batch_solve_f64(cholespy_gpu_solver *batch, const int *n_rhs, const double *b,
+                             double *x, void *stream);
+
 /* Selects the schedule used by later solves (default CHOLESPY_PATH_BLOCKS).
  * No counterpart in the reference, which has the row schedule only. */
 int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);

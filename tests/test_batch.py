@@ -1,0 +1,64 @@
+"""Batched solves: many independent systems, one launch per pass. Each
+system's slice of the packed solution is checked against the oracle on that
+system's own factor, in the solver's precision."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from cholespy_amd import BatchedSolverD, BatchedSolverF, CholeskySolverD, MatrixType, workloads
+from oracle import oracle
+
+
+def test_batch_generator_shapes():
+    systems = workloads.batch_systems(count=16, seed=1, n_min=100, n_max=2000)
+    for n, values, idx, k in systems:
+        assert 1 <= k <= 128 and idx.shape[1] == len(values) and idx.max() < n
+
+
+def test_batch_constructor_factors_each_system():
+    """CPU side: the batch factors each system like a single solver."""
+    systems = workloads.batch_systems(count=3, seed=2, n_min=50, n_max=300)
+    if torch.cuda.device_count() > 0:
+        pytest.skip("covered by the GPU test on a GPU box")
+    with pytest.raises(RuntimeError):   # no device here: fails loudly, no CPU fallback
+        BatchedSolverF([(n, idx[0], idx[1], v, MatrixType.COO) for n, v, idx, k in systems])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cls,dtype,tol", [(BatchedSolverF, np.float32, 2e-5), (BatchedSolverD, np.float64, 1e-12)])
+def test_batch_matches_oracle(cls, dtype, tol):
+    systems = workloads.batch_systems(count=24, seed=3, n_min=50, n_max=3000)
+    batch = cls([(n, idx[0], idx[1], v, MatrixType.COO) for n, v, idx, k in systems])
+    rng = np.random.default_rng(4)
+    for ks in ([k for *_, k in systems], [1 + (i % 5) for i in range(len(systems))]):
+        bs = [rng.uniform(-1, 1, (n, k)).astype(dtype) for (n, v, idx, _), k in zip(systems, ks)]
+        b = torch.tensor(np.concatenate([x.ravel() for x in bs]), device="cuda")
+        x = torch.zeros_like(b)
+        batch.solve(ks, b, x)
+        torch.cuda.synchronize()
+        batch._check()
+        xs = x.cpu().numpy()
+        off = 0
+        for i, ((n, v, idx, _), k) in enumerate(zip(systems, ks)):
+            got = xs[off:off + n * k].reshape(n, k)
+            off += n * k
+            ref = oracle.solve(batch._factor(i), bs[i], dtype)
+            err = np.abs(got - ref).max() / np.abs(ref).max()
+            assert err < tol, (i, n, k, err)
+
+
+@pytest.mark.gpu
+def test_batch_of_one_equals_single_solver():
+    n = 4096
+    v, idx = workloads.grid2d_laplacian(64, 64, 1e-2)
+    batch = BatchedSolverD([(n, idx[0], idx[1], v, MatrixType.COO)])
+    single = CholeskySolverD(n, torch.tensor(idx[0], device="cuda"), torch.tensor(idx[1], device="cuda"),
+                             torch.tensor(v, device="cuda"), MatrixType.COO)
+    b = torch.rand(n, 7, device="cuda", dtype=torch.float64)
+    x1 = torch.zeros_like(b)
+    x2 = torch.zeros_like(b)
+    single.solve(b, x1)
+    batch.solve([7], b.reshape(-1), x2.reshape(-1))
+    torch.cuda.synchronize()
+    assert torch.equal(x1, x2)
