@@ -201,6 +201,29 @@ def main():
     torch.cuda.synchronize()
     solver._check()
 
+    # The reference's own published benchmark (tests/Benchmark.ipynb:238-275,
+    # README.md:24-33): icosphere(6), A = I + 20 (D - W) (the last lambda of
+    # the timed loop), 128 right-hand sides U[0,1), CholeskySolverF.solve;
+    # cholespy on a TITAN RTX: 6.42 ms per solve.
+    ref_bench = None
+    try:
+        n6, f6 = workloads.icosphere(6)
+        v6, i6 = workloads.mesh_laplacian(n6, f6, 20.0)
+        s6 = CholeskySolverF(n6, torch.tensor(i6[0], device=dev), torch.tensor(i6[1], device=dev),
+                             torch.tensor(v6, device=dev), MatrixType.COO)
+        b6 = torch.rand(n6, 128, device=dev)
+        x6 = torch.zeros_like(b6)
+        ms6 = time_events(lambda: s6.solve(b6, x6), 20)
+        ms6z = time_events(lambda: (torch.zeros_like(b6), s6.solve(b6, x6)), 20)
+        s6._check()
+        ref_bench = {"workload": "icosphere(6), n=40962, lambda=20, k=128, float32", "ms_solve": ms6,
+                     "ms_solve_with_zeros_like": ms6z, "reference_ms_titan_rtx": 6.42,
+                     "speedup_vs_reference": 6.42 / ms6z}
+        log(f"reference benchmark config: {ms6:.3f} ms/solve ({ms6z:.3f} with zeros_like) vs 6.42 ms")
+        del s6
+    except Exception as e:  # pragma: no cover - reported, not fatal
+        log(f"reference benchmark config failed: {e}")
+
     # CPU baseline: oracle FP64 simplicial solve, one thread, ~cpu_seconds of work.
     cpu = None
     if not args.no_cpu:
@@ -242,6 +265,7 @@ def main():
             "rel_errs": {name: r["rel_err"] for name, r in results.items()},
             "vs_plain": results["plain"]["ms"] / ms_step,
             "block_plan": plan,
+            "reference_benchmark": ref_bench,
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -64,6 +64,7 @@ class SolveFunction(torch.autograd.Function):
     @staticmethod
     def backward(ctx, gx):
         rows, cols, x = ctx.saved_tensors
+        x = x.detach()   # the saved output comes back attached to the graph
         lam = torch.empty_like(x)
         ctx.solver.solve(gx.detach().to(x.dtype).contiguous(), lam)
         gb = lam if ctx.needs_input_grad[0] else None

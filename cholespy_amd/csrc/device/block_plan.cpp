@@ -174,24 +174,38 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
-    for (int b : order) {
-        const int rows = (start[b + 1] - start[b]) + (P.rptr[b + 1] - P.rptr[b]);
-        for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows) {
-            P.fwd_task.push_back(b);
-            P.fwd_task.push_back(r0);
-            P.fwd_task.push_back(std::min(rows, r0 + opt.fwd_rows));
+    auto push = [](std::vector<int> &v, int b, int f, int l, int kind) {
+        v.push_back(b);
+        v.push_back(f);
+        v.push_back(l);
+        v.push_back(kind);
+    };
+    for (size_t lo = 0; lo < order.size();) {
+        // One level at a time: the gather tasks of its large blocks first,
+        // then the product tasks, so a product task's r_B is gathered by
+        // tasks with smaller tickets.
+        size_t hi = lo;
+        while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q], m = start[b + 1] - start[b];
+            if (m < opt.split_gather) continue;
+            for (int r0 = 0; r0 < m; r0 += opt.fwd_rows) push(P.fwd_task, b, r0, std::min(m, r0 + opt.fwd_rows), 1);
         }
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q], m = start[b + 1] - start[b];
+            const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
+            const int kind = m < opt.split_gather ? 0 : 2;
+            for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows)
+                push(P.fwd_task, b, r0, std::min(rows, r0 + opt.fwd_rows), kind);
+        }
+        lo = hi;
     }
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
     for (int b : order) {
         const int m = start[b + 1] - start[b];
-        for (int c0 = 0; c0 < m; c0 += opt.bwd_cols) {
-            P.bwd_task.push_back(b);
-            P.bwd_task.push_back(c0);
-            P.bwd_task.push_back(std::min(m, c0 + opt.bwd_cols));
-        }
+        for (int c0 = 0; c0 < m; c0 += opt.bwd_cols) push(P.bwd_task, b, c0, std::min(m, c0 + opt.bwd_cols), 0);
     }
     return P;
 }

@@ -20,10 +20,11 @@
 namespace cholespy_amd {
 
 struct BlockPlanOptions {
-    int max_block = 256;    // rows per block (dense inverse is m x m)
-    int leaf_max = 16;      // whole subtrees up to this size become one block
+    int max_block = 1024;   // rows per block (dense inverse is m x m)
+    int leaf_max = 32;      // whole subtrees up to this size become one block
     int fwd_rows = 64;      // rows of M_B per forward task
     int bwd_cols = 32;      // columns of M_B per backward task
+    int split_gather = 65;  // blocks with m >= this gather r_B in separate tasks
 };
 
 struct BlockPlan {
@@ -42,9 +43,12 @@ struct BlockPlan {
     std::vector<double> mfwd, mbwd;
     // Block level in the block tree: forward level (height) and backward level (depth).
     std::vector<int> level_fwd, level_bwd;
-    // Tasks in schedule order: (block, first, last) with rows of M_B
-    // (forward) or columns of M_B (backward) in [first, last).
-    std::vector<int> fwd_task, bwd_task;   // 3 ints per task
+    // Tasks in schedule order, 4 ints each: (block, first, last, kind).
+    // Forward kinds: 0 = gather r_B and multiply rows [first, last) of M_B;
+    // 1 = gather rows [first, last) of r_B only (published for kind 2);
+    // 2 = multiply rows [first, last) of M_B by the published r_B.
+    // Backward (kind 0): columns [first, last) of M_B.
+    std::vector<int> fwd_task, bwd_task;
     int max_m = 0, max_rows = 0;           // largest m and m + |R|
     int64_t nnz_l = 0;
 };

@@ -115,6 +115,7 @@ struct cholespy_gpu_solver {
     int *status = nullptr;                // [0] error code, [1] abort flag
     void *y = nullptr, *z = nullptr;      // per-solve work buffers (n x k)
     void *part = nullptr;                 // block path partial slots (nslots x k)
+    void *rbuf = nullptr;                 // block path gathered right-hand sides (n x k)
     int k_cur = 0;                        // layout the work buffers hold
     int path = CHOLESPY_PATH_BLOCKS;
     int buffers_path = -1;                // path whose invariants the buffers hold
@@ -128,7 +129,7 @@ struct cholespy_gpu_solver {
     float *r32 = nullptr;                 // FP32 right-hand side of correction solves
     size_t r32_elems = 0;
     int grid_rows[2][2] = {{0, 0}, {0, 0}};   // [lower][k > 64]
-    int grid_blocks[2] = {0, 0};              // [forward]
+    int grid_blocks[2][2] = {{0, 0}, {0, 0}};   // [forward][wide tile]
     // Ticket queues per kernel: [0..1] analysis upper/lower, [2..3] row pass
     // backward/forward, [4..5] block pass backward/forward.
     unsigned int *qctr = nullptr;
@@ -168,8 +169,8 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
     d.nblocks = P.nblocks;
     d.nslots = P.rptr[P.nblocks];
     d.max_rows = P.max_rows;
-    d.n_fwd_tasks = (int)(P.fwd_task.size() / 3);
-    d.n_bwd_tasks = (int)(P.bwd_task.size() / 3);
+    d.n_fwd_tasks = (int)(P.fwd_task.size() / 4);
+    d.n_bwd_tasks = (int)(P.bwd_task.size() / 4);
     d.levels_fwd = d.levels_bwd = 0;
     for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
     for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
@@ -195,7 +196,7 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
         HIP_TRY(upload(reinterpret_cast<double **>(&d.mfwd), P.mfwd.data(), me));
         HIP_TRY(upload(reinterpret_cast<double **>(&d.mbwd), P.mbwd.data(), me));
     }
-    s->grid_blocks[0] = s->grid_blocks[1] = 0;
+    s->grid_blocks[0][0] = s->grid_blocks[0][1] = s->grid_blocks[1][0] = s->grid_blocks[1][1] = 0;
     s->buffers_path = -1;
     return CHOLESPY_OK;
 }
@@ -288,6 +289,7 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->y);
     dev_free(s->z);
     dev_free(s->part);
+    dev_free(s->rbuf);
     dev_free(s->arowptr);
     dev_free(s->acols);
     dev_free(s->avals);
@@ -453,6 +455,7 @@ int cholespy_solver_check(cholespy_gpu_solver *s) {
             HIP_TRY(launch_fill_sentinel(s->z, s->work_elems, s->fbytes, nullptr));
             if (s->part && s->part_elems)
                 HIP_TRY(launch_fill_sentinel(s->part, s->part_elems, s->fbytes, nullptr));
+            if (s->rbuf) HIP_TRY(launch_fill_sentinel(s->rbuf, s->work_elems, s->fbytes, nullptr));
             HIP_TRY(hipDeviceSynchronize());
         }
         return fail(CHOLESPY_ERR_TIMEOUT, "a solve kernel exceeded its bounded wait");
@@ -472,8 +475,10 @@ int prepare_buffers(cholespy_gpu_solver *s, int k, hipStream_t st) {
     if (elems > s->work_elems) {
         dev_free(s->y);
         dev_free(s->z);
+        dev_free(s->rbuf);
         HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), elems * (size_t)fbytes));
         HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), elems * (size_t)fbytes));
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->rbuf), elems * (size_t)fbytes));
         s->work_elems = elems;
         s->k_cur = 0;
     }
@@ -488,6 +493,7 @@ int prepare_buffers(cholespy_gpu_solver *s, int k, hipStream_t st) {
         // Buffers are indexed row * k + col: refill them for the new layout.
         HIP_TRY(launch_fill_sentinel(s->y, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->z, elems, fbytes, st));
+        HIP_TRY(launch_fill_sentinel(s->rbuf, elems, fbytes, st));
         if (s->part && pe) HIP_TRY(launch_fill_sentinel(s->part, pe, fbytes, st));
         s->k_cur = k;
         s->buffers_path = s->path;
@@ -530,6 +536,9 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     const DeviceBlockPlan &d = s->plan;
     BlockPassBuffers bf;
     bf.k = k;
+    bf.max_k = k;
+    if (s->is_batch)
+        for (int v : s->last_k) bf.max_k = std::max(bf.max_k, v);
     bf.max_rows = d.max_rows;
     bf.start = d.start;
     bf.rptr = d.rptr;
@@ -543,6 +552,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.y = s->y;
     bf.z = s->z;
     bf.part = s->part;
+    bf.rbuf = s->rbuf;
     bf.status = s->status;
     bf.xmode = xmode;
     if (s->is_batch) {
@@ -558,10 +568,11 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
         bf.task = forward ? d.ftask : d.btask;
         bf.M = forward ? d.mfwd : d.mbwd;
         bf.trace = s->trace[forward ? 1 : 0];
-        int &grid = s->grid_blocks[forward ? 1 : 0];
+        const int tile = block_tile(fbytes, bf.max_k);
+        int &grid = s->grid_blocks[forward ? 1 : 0][tile > 4 ? 1 : 0];
         if (grid == 0) {
-            const size_t lds = block_pass_lds(fbytes, d.max_rows);
-            const int cap = max_coresident_blocks(block_pass_kernel(fbytes, forward), 256, lds);
+            const size_t lds = block_pass_lds(fbytes, d.max_rows, tile);
+            const int cap = max_coresident_blocks(block_pass_kernel(fbytes, forward, tile), 256, lds);
             if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for the block solve");
             grid = std::max(1, std::min(cap, bf.ntasks));
         }
@@ -819,10 +830,10 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         G.mbwd.insert(G.mbwd.end(), P.mbwd.begin(), P.mbwd.end());
         for (int r = 0; r < s->sys_n[i]; ++r) G.contrib_ptr.push_back(P.contrib_ptr[r] + C0);
         for (int v : P.contrib_slot) G.contrib_slot.push_back(v + S0);
-        for (size_t t = 0; t < P.fwd_task.size() / 3; ++t)
-            ftasks.push_back({P.level_fwd[P.fwd_task[3 * t]], i, (int)t});
-        for (size_t t = 0; t < P.bwd_task.size() / 3; ++t)
-            btasks.push_back({P.level_bwd[P.bwd_task[3 * t]], i, (int)t});
+        for (size_t t = 0; t < P.fwd_task.size() / 4; ++t)
+            ftasks.push_back({P.level_fwd[P.fwd_task[4 * t]], i, (int)t});
+        for (size_t t = 0; t < P.bwd_task.size() / 4; ++t)
+            btasks.push_back({P.level_bwd[P.bwd_task[4 * t]], i, (int)t});
         G.max_m = std::max(G.max_m, P.max_m);
         G.max_rows = std::max(G.max_rows, P.max_rows);
         G.nnz_l += P.nnz_l;
@@ -843,9 +854,10 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         std::stable_sort(refs.begin(), refs.end(), [](const TaskRef &x, const TaskRef &y) { return x.level < y.level; });
         for (const TaskRef &r : refs) {
             const std::vector<int> &src = fwd ? plans[r.sys].fwd_task : plans[r.sys].bwd_task;
-            out.push_back(src[3 * r.idx] + blk0[r.sys]);
-            out.push_back(src[3 * r.idx + 1]);
-            out.push_back(src[3 * r.idx + 2]);
+            out.push_back(src[4 * r.idx] + blk0[r.sys]);
+            out.push_back(src[4 * r.idx + 1]);
+            out.push_back(src[4 * r.idx + 2]);
+            out.push_back(src[4 * r.idx + 3]);
         }
     };
     emit(ftasks, true, G.fwd_task);
@@ -904,8 +916,10 @@ int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void
         if (elems > s->work_elems) {
             dev_free(s->y);
             dev_free(s->z);
+            dev_free(s->rbuf);
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), elems * (size_t)fbytes));
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), elems * (size_t)fbytes));
+            HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->rbuf), elems * (size_t)fbytes));
             s->work_elems = elems;
         }
         if (pe > s->part_elems) {
@@ -915,6 +929,7 @@ int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void
         }
         HIP_TRY(launch_fill_sentinel(s->y, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->z, elems, fbytes, st));
+        HIP_TRY(launch_fill_sentinel(s->rbuf, elems, fbytes, st));
         if (pe) HIP_TRY(launch_fill_sentinel(s->part, pe, fbytes, st));
         s->last_k = ks;
         s->buffers_path = CHOLESPY_PATH_BLOCKS;

@@ -51,10 +51,13 @@ hipError_t launch_row_pass(int float_bytes, bool lower, const RowPassBuffers &bu
                            QueueState &queues, hipStream_t stream);
 
 // Partitioned-inverse block passes (block_plan.hpp / trsv_blocks.hip).
-constexpr int kBlockKT = 4;   // right-hand-side columns per LDS tile
+// Right-hand-side columns per LDS tile: narrow for k <= 4, wide otherwise
+// (fewer passes over M_B per task); the wide tile halves in FP64.
+inline int block_tile(int float_bytes, int k) { return k <= 4 ? 4 : (float_bytes == 4 ? 16 : 8); }
 struct BlockPassBuffers {
     int ntasks = 0, k = 0, max_rows = 0;
-    const int *task = nullptr;     // (block, first, last) per task, in ticket order
+    int max_k = 0;                 // largest right-hand-side count of any system (tile choice)
+    const int *task = nullptr;     // (block, first, last, kind) per task, in ticket order
     const int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
     const int64_t *moff = nullptr;
     const void *M = nullptr;       // forward or backward layout of M_B
@@ -62,6 +65,7 @@ struct BlockPassBuffers {
     const int *perm = nullptr;
     const void *b = nullptr;
     void *y = nullptr, *z = nullptr, *part = nullptr, *x = nullptr;
+    void *rbuf = nullptr;          // gathered r_B of large blocks (n x k, sentinel-protected)
     int *status = nullptr;
     // Backward epilogue: 0 = x (solver precision) = z; 1 = x (double) = z;
     // 2 = x (double) += z (refinement update).
@@ -71,8 +75,8 @@ struct BlockPassBuffers {
     const int *blk_sys = nullptr, *sys_k = nullptr;
     const long long *sys_vbase = nullptr, *sys_pbase = nullptr;
 };
-size_t block_pass_lds(int float_bytes, int max_rows);
-const void *block_pass_kernel(int float_bytes, bool forward);
+size_t block_pass_lds(int float_bytes, int max_rows, int tile);
+const void *block_pass_kernel(int float_bytes, bool forward, int tile);
 hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffers &buf, int grid,
                              QueueState &queues, hipStream_t stream);
 

@@ -53,7 +53,7 @@ struct BlockPassArgs {
     const int *cslot;
     const int *perm;
     const F *b;
-    F *y, *z, *part;
+    F *y, *z, *part, *rbuf;
     void *x;
     int *status;
     int xmode;
@@ -114,7 +114,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-        const int blk = a.task[3 * tk], first = a.task[3 * tk + 1], last = a.task[3 * tk + 2];
+        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
+        const int kind = a.task[4 * tk + 3];
         const int a0 = a.start[blk];
         const int m = a.start[blk + 1] - a0;
         const int R = a.rptr[blk + 1] - a.rptr[blk];
@@ -122,14 +123,17 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const F *M = a.M + a.moff[blk];
         const VecMap vm = vec_map(a, blk);
         const int k = vm.k;
-        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
         const F *bv = a.b + vm.vbase;
+        // Rows of r_B this task gathers: all (kind 0), its slice (kind 1), none (kind 2).
+        const int g0 = kind == 1 ? first : 0;
+        const int g1 = kind == 1 ? last : (kind == 0 ? m : 0);
         for (int t0 = 0; t0 < k; t0 += KT) {
             const int kt = min(KT, k - t0);
             // r_B: each thread owns up to kPairs (row, column) pairs and
             // walks their partial-slot lists together, 4 slots per pair per
             // round, so up to 4 * kPairs slot loads are in flight at once.
-            for (int base = tid; base < m * KT; base += kTaskThreads * kPairs) {
+            for (int base = g0 * KT + tid; base < g1 * KT; base += kTaskThreads * kPairs) {
                 int p0[kPairs], p1[kPairs], col[kPairs];
                 F v[kPairs];
 #pragma unroll
@@ -139,7 +143,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     p0[q] = p1[q] = 0;
                     col[q] = t0 + t;
                     v[q] = F(0);
-                    if (idx < m * KT && t < kt) {
+                    if (idx < g1 * KT && t < kt) {
                         const int row = a0 + i;
                         v[q] = bv[(long long)a.perm[row] * k + col[q]];
                         p0[q] = a.cptr[row];
@@ -173,11 +177,42 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
 #pragma unroll
                 for (int q = 0; q < kPairs; ++q) {
                     const int idx = base + q * kTaskThreads;
-                    if (idx < m * KT) rv[idx] = v[q];
+                    if (idx >= g1 * KT) continue;
+                    if (kind == 1) {
+                        if (idx % KT < kt) publish(&rb[(long long)(a0 + idx / KT) * k + col[q]], v[q]);
+                    } else {
+                        rv[idx] = v[q];
+                    }
+                }
+            }
+            if (kind == 2) {
+                // r_B from the gather tasks of this block (published, waited on).
+                for (int base = tid; base < m * KT; base += kTaskThreads * kGatherUnroll) {
+                    U u[kGatherUnroll];
+#pragma unroll
+                    for (int j = 0; j < kGatherUnroll; ++j) {
+                        const int idx = base + j * kTaskThreads;
+                        const int i = idx / KT, t = idx % KT;
+                        u[j] = (idx < m * KT && t < kt) ? load_bits(&rb[(long long)(a0 + i) * k + t0 + t]) : U(0);
+                    }
+#pragma unroll
+                    for (int j = 0; j < kGatherUnroll; ++j) {
+                        const int idx = base + j * kTaskThreads;
+                        if (idx >= m * KT) break;
+                        const int i = idx / KT, t = idx % KT;
+                        F v = from_bits<F>(u[j]);
+                        if (t < kt && u[j] == Bits<F>::sentinel)
+                            v = wait_value(&rb[(long long)(a0 + i) * k + t0 + t], a.status);
+                        rv[idx] = v;
+                    }
                 }
             }
             __syncthreads();
             if (a.trace && tid == 0 && t0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+            if (kind == 1) {   // gather-only task: nothing to multiply
+                __syncthreads();
+                continue;
+            }
             const int rho = first + lr;
             F acc[KT];
 #pragma unroll
@@ -259,7 +294,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-        const int blk = a.task[3 * tk], first = a.task[3 * tk + 1], last = a.task[3 * tk + 2];
+        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
         const int a0 = a.start[blk];
         const int m = a.start[blk + 1] - a0;
         const int r0 = a.rptr[blk];
@@ -358,6 +393,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         if (first == 0) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
             for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
+            F *rb = a.rbuf + vm.vbase;
+            for (int idx = tid; idx < m * k; idx += kTaskThreads) rb[(long long)a0 * k + idx] = sent;
         }
         if (a.trace && tid == 0) {
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
@@ -391,6 +428,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.y = static_cast<F *>(bf.y);
     a.z = static_cast<F *>(bf.z);
     a.part = static_cast<F *>(bf.part);
+    a.rbuf = static_cast<F *>(bf.rbuf);
     a.x = bf.x;
     a.status = bf.status;
     a.xmode = bf.xmode;
@@ -401,23 +439,31 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.sys_pbase = bf.sys_pbase;
     queues.fill(a.qa);
     void *args[] = {&a};
-    const size_t lds = block_pass_lds(sizeof(F), bf.max_rows);
-    return hipLaunchKernel(block_pass_kernel(sizeof(F), forward), dim3(grid), dim3(kTaskThreads), args,
+    const int tile = block_tile(sizeof(F), bf.max_k);
+    const size_t lds = block_pass_lds(sizeof(F), bf.max_rows, tile);
+    return hipLaunchKernel(block_pass_kernel(sizeof(F), forward, tile), dim3(grid), dim3(kTaskThreads), args,
                            lds, stream);
 }
 
 }  // namespace
 
-size_t block_pass_lds(int float_bytes, int max_rows) {
-    return (size_t)(max_rows + kTaskThreads) * kBlockKT * (size_t)float_bytes;
+size_t block_pass_lds(int float_bytes, int max_rows, int tile) {
+    return (size_t)(max_rows + kTaskThreads) * tile * (size_t)float_bytes;
 }
 
-const void *block_pass_kernel(int float_bytes, bool forward) {
-    if (float_bytes == 4)
-        return forward ? reinterpret_cast<const void *>(&block_forward<float, kBlockKT>)
-                       : reinterpret_cast<const void *>(&block_backward<float, kBlockKT>);
-    return forward ? reinterpret_cast<const void *>(&block_forward<double, kBlockKT>)
-                   : reinterpret_cast<const void *>(&block_backward<double, kBlockKT>);
+const void *block_pass_kernel(int float_bytes, bool forward, int tile) {
+    if (float_bytes == 4) {
+        if (tile == 4)
+            return forward ? reinterpret_cast<const void *>(&block_forward<float, 4>)
+                           : reinterpret_cast<const void *>(&block_backward<float, 4>);
+        return forward ? reinterpret_cast<const void *>(&block_forward<float, 16>)
+                       : reinterpret_cast<const void *>(&block_backward<float, 16>);
+    }
+    if (tile == 4)
+        return forward ? reinterpret_cast<const void *>(&block_forward<double, 4>)
+                       : reinterpret_cast<const void *>(&block_backward<double, 4>);
+    return forward ? reinterpret_cast<const void *>(&block_forward<double, 8>)
+                   : reinterpret_cast<const void *>(&block_backward<double, 8>);
 }
 
 hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffers &buf, int grid,

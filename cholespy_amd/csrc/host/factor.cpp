@@ -150,6 +150,7 @@ Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
             }
         NDOptions nd;
         nd.leaf_size = opt.nd_leaf_size;
+        nd.min_balance = opt.nd_min_balance;
         perm0 = nested_dissection(g, nd);
     }
 

@@ -29,6 +29,7 @@ struct FactorOptions {
     // Use the identity ordering instead of nested dissection (tests only).
     bool natural = false;
     int nd_leaf_size = 48;
+    double nd_min_balance = 0.25;
 };
 
 // Throws std::invalid_argument("Matrix is not positive definite!") when a
