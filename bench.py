@@ -6,20 +6,32 @@ sides = vertex positions + N(0, 0.01) noise (seed 0), factored once on the
 host (nested dissection + simplicial LL').
 
 Paths timed in the same process, inputs resident in HBM:
-  refined  FP32 block solve + 2 FP64 refinement steps (the headline when it
-           reaches <= 1e-10 relative error against the FP64 host solve)
+  refined  FP32 block solve + 2 FP64 refinement steps
   f64      FP64 block solve
+  f32      FP32 block solve alone (reported, below the accuracy bar)
   plain    FP64 row-per-wavefront level-scheduled solve (the reference's
            algorithm on gfx950; the "vs plain" denominator)
-A step is one solve of the 3 right-hand sides; `value` = 3 * steps / time.
+  plain32  the same in FP32
+The headline is the fastest path whose relative error against the FP64
+host solve is <= 1e-10. A step is one solve of the 3 right-hand sides;
+`value` = 3 * steps / time. Every path is also timed cold: a 512 MiB
+buffer is overwritten before each solve so the factor is not served from
+the 256 MB Infinity Cache.
 
-roofline: the dominant kernel of the headline path (the FP32 block pass that
+roofline: the dominant kernel of the headline path (the block pass that
 takes longer), timed with HIP events recorded between the two passes of
 complete solves on the stream they run on (the null stream); achieved =
-algorithmic bytes of that pass / its average time.
+algorithmic bytes of that pass / its average time. traffic = the HBM bytes
+rocprofv3 counted for that kernel on this workload (FETCH_SIZE and
+WRITE_SIZE, scaled by the counts of a copy of known size), read from the
+summary kept beside this file in profiles/ when it names the same kernel.
 cpu_baseline: the oracle's FP64 simplicial solve (oracle/oracle.c,
 oracle_host_solve: what cholmod_solve does for a simplicial LL' factor) on
 the same factor, one thread, bounded to ~10 s of work.
+
+--profile runs only the headline path for `steps` solves and then a plain
+256 MiB device copy (the known-size kernel that scales the counters); it is
+the program the rocprofv3 counter passes run.
 """
 from __future__ import annotations
 
@@ -67,6 +79,21 @@ def time_events(fn, reps: int, warm: int = 3) -> float:
     return e0.elapsed_time(e1) / reps
 
 
+def measured_traffic(kernel: str, workload: str, k: int):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 counter summary (profiles/traffic.json), or None when the
+    summary does not cover this kernel and workload."""
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        entries = json.loads(path.read_text())
+    except (OSError, ValueError):
+        return None
+    for e in entries:
+        if e.get("kernel") == kernel and e.get("workload") == workload and e.get("k") == k:
+            return e.get("bytes_per_launch")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +104,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--workload", choices=["smoothing", "batch"], default="smoothing")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--profile", action="store_true",
+                    help="headline path only, then a 256 MiB copy (for rocprofv3 counter passes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the solve phase is single-device); "
@@ -126,30 +155,54 @@ def main():
     def run_f64():
         sd.solve(b64, x64)
 
+    def run_f32():
+        sf.solve(b32, x32)
+
+    # Cold runs: overwrite a 512 MiB buffer (twice the Infinity Cache) before
+    # each solve; only the solve is inside the events.
+    flush = torch.empty(128 * 1024 * 1024, device=dev)
+
+    def time_cold(fn, reps):
+        ts = []
+        for _ in range(reps):
+            flush.fill_(1.0)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return float(np.median(ts))
+
+    paths = {"refined": (sf, "blocks", run_refined, x64),
+             "f64": (sd, "blocks", run_f64, x64),
+             "f32": (sf, "blocks", run_f32, x32),
+             "plain": (sd, "rows", run_f64, x64),
+             "plain32": (sf, "rows", run_f32, x32)}
+    if args.profile:
+        paths = {"f64": paths["f64"], "refined": paths["refined"]}
     results = {}
-    sd._set_path(P["blocks"])
-    sf._set_path(P["blocks"])
-    for name, fn in (("refined", run_refined), ("f64", run_f64)):
+    for name, (solver, path, fn, out) in paths.items():
+        solver._set_path(P[path])
         fn()
         torch.cuda.synchronize()
-        (sf if name == "refined" else sd)._check()
-        err = rel_err(x64)
-        ms = time_events(fn, max(10, args.steps // 4))
-        results[name] = {"ms": ms, "rel_err": err}
-        log(f"{name:8s} {ms:8.4f} ms/solve  rel.err {err:.2e}")
-    sd._set_path(P["rows"])
-    run_f64()
-    torch.cuda.synchronize()
-    sd._check()
-    results["plain"] = {"ms": time_events(run_f64, 5, warm=1), "rel_err": rel_err(x64)}
-    log(f"plain    {results['plain']['ms']:8.4f} ms/solve  rel.err {results['plain']['rel_err']:.2e}")
-    sd._set_path(P["blocks"])
+        solver._check()
+        err = rel_err(out)
+        slow = path == "rows"
+        reps = 5 if (slow or args.profile) else max(10, args.steps // 4)
+        ms = time_events(fn, reps, warm=1 if slow else 3)
+        cold = None if args.profile else time_cold(fn, 3 if slow else 20)
+        solver._set_path(P["blocks"])
+        results[name] = {"ms": ms, "ms_cold": cold, "rel_err": err}
+        log(f"{name:8s} {ms:8.4f} ms/solve (cold {cold}) rel.err {err:.2e}")
 
-    headline = "refined" if results["refined"]["rel_err"] <= TOL else "f64"
-    if results[headline]["rel_err"] > TOL:
+    ok = [p for p in ("f64", "refined") if results[p]["rel_err"] <= TOL]
+    if not ok:
         log("no path reached the accuracy bar")
-    fn = run_refined if headline == "refined" else run_f64
-    solver = sf if headline == "refined" else sd
+        ok = ["f64"]
+    headline = min(ok, key=lambda p: results[p]["ms"])
+    solver, _, fn, xout = paths[headline]
 
     # Timed region: exactly `steps` solves bracketed by synchronisations.
     for _ in range(args.warmup):
@@ -166,32 +219,44 @@ def main():
     wall = time.perf_counter() - w0
     solver._check()
     ms_step = e0.elapsed_time(e1) / args.steps
-    final_err = rel_err(x64)
+    final_err = rel_err(xout)
     value = k * 1000.0 / ms_step
-    log(f"timed {args.steps} steps: {ms_step:.4f} ms/step (wall {wall * 1000 / args.steps:.4f}), "
-        f"rel.err {final_err:.2e}")
+    log(f"timed {args.steps} steps ({headline}): {ms_step:.4f} ms/step "
+        f"(wall {wall * 1000 / args.steps:.4f}), rel.err {final_err:.2e}")
 
-    # Roofline of the dominant kernel: each FP32 block pass alone.
+    if args.profile:
+        # Known-size kernel for scaling FETCH_SIZE / WRITE_SIZE: 256 MiB read,
+        # 256 MiB written.
+        src = torch.ones(64 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": headline, "ms_per_step": ms_step, "rel_err": final_err}))
+        return
+
+    # Roofline of the dominant kernel: each block pass of the headline path.
+    # The refined path's passes are FP32 block passes.
     s_bytes = 4 if headline == "refined" else 8
     pb = pass_bytes(s_bytes, n, nnz_l, k)
+    solve_bytes = 2 * pb   # B_solve: both triangles, b read, y written+read, x written
     rhs_pass = b32 if headline == "refined" else b64
     out_pass = x32 if headline == "refined" else x64
     # Each pass timed inside complete solves (forward then backward, events
     # between them), so every dependency wait of the real schedule is paid.
-    import torch as _t
     reps = 50
-    ev = [[_t.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(reps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(reps)]
     for _ in range(3):
         solver._solve_pass(0, rhs_pass, out_pass)
         solver._solve_pass(1, rhs_pass, out_pass)
-    _t.cuda.synchronize()
+    torch.cuda.synchronize()
     for e in ev:
         e[0].record()
         solver._solve_pass(0, rhs_pass, out_pass)
         e[1].record()
         solver._solve_pass(1, rhs_pass, out_pass)
         e[2].record()
-    _t.cuda.synchronize()
+    torch.cuda.synchronize()
     pass_ms = [float(np.mean([e[0].elapsed_time(e[1]) for e in ev])),
                float(np.mean([e[1].elapsed_time(e[2]) for e in ev]))]
     dom = int(np.argmax(pass_ms))
@@ -200,6 +265,10 @@ def main():
         f"algorithmic bytes per pass {pb / 1e6:.1f} MB")
     torch.cuda.synchronize()
     solver._check()
+    kernel = ("block_forward" if dom == 0 else "block_backward") + \
+             ("<float,4>" if s_bytes == 4 else "<double,4>")
+    workload_name = f"mesh smoothing I+10(D-W) on icosphere({args.level})"
+    traffic = measured_traffic(kernel, workload_name, k)
 
     # The reference's own published benchmark (tests/Benchmark.ipynb:238-275,
     # README.md:24-33): icosphere(6), A = I + 20 (D - W) (the last lambda of
@@ -255,28 +324,30 @@ def main():
         "dtype": "f32+f64" if headline == "refined" else "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"mesh smoothing I+10(D-W) on icosphere({args.level})",
+            "workload": workload_name,
             "n": n, "nnz_A": nnz_a, "nnz_L": nnz_l, "k": k,
             "path": "blocks FP32 + FP64 refinement" if headline == "refined" else "blocks FP64",
             "refine_iters": REFINE_ITERS if headline == "refined" else 0,
             "rel_err": final_err,
             "solves_per_s": 1000.0 / ms_step,
             "ms": {name: r["ms"] for name, r in results.items()},
+            "ms_cold": {name: r["ms_cold"] for name, r in results.items()},
             "rel_errs": {name: r["rel_err"] for name, r in results.items()},
             "vs_plain": results["plain"]["ms"] / ms_step,
+            "refined_vs_plain": results["plain"]["ms"] / results["refined"]["ms"],
+            "solve_roofline_frac": solve_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "block_plan": plan,
             "reference_benchmark": ref_bench,
             "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": ("block_forward" if dom == 0 else "block_backward") +
-                      ("<float>" if s_bytes == 4 else "<double>"),
+            "kernel": kernel,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "pass_us": [pass_ms[0] * 1000, pass_ms[1] * 1000],
             "bytes_per_launch": pb,
         },

@@ -62,3 +62,41 @@ def test_batch_of_one_equals_single_solver():
     batch.solve([7], b.reshape(-1), x2.reshape(-1))
     torch.cuda.synchronize()
     assert torch.equal(x1, x2)
+
+
+@pytest.mark.gpu
+def test_batch_graph_replays_match_eager():
+    """One batched solve captured in a hipGraph and replayed three times,
+    with new right-hand sides written eagerly between the replays; every
+    replay equals the eager solve of the same right-hand sides bit for bit
+    (the kernels reset their own queue counters and sentinels, so a replay
+    needs no memset node)."""
+    systems = workloads.batch_systems(count=12, seed=5, n_min=200, n_max=4000)
+    ks = [k for *_, k in systems]
+    batch = BatchedSolverF([(n, idx[0], idx[1], v, MatrixType.COO) for n, v, idx, k in systems])
+    total = sum(n * k for (n, *_), k in zip(systems, ks))
+    gen = torch.Generator(device="cuda").manual_seed(6)
+    b = torch.rand(total, device="cuda", generator=gen)
+    x = torch.zeros_like(b)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        batch.solve(ks, b, x, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        batch.solve(ks, b, x, torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        b.copy_(torch.rand(total, device="cuda", generator=gen))
+        x.fill_(0.0)
+        g.replay()
+        torch.cuda.synchronize()
+        batch._check()
+        got = x.clone()
+        x_eager = torch.zeros_like(b)
+        batch.solve(ks, b, x_eager)
+        torch.cuda.synchronize()
+        batch._check()
+        assert torch.equal(got, x_eager), rep
