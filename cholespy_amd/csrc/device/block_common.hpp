@@ -1,0 +1,192 @@
+// Task machinery shared by the block passes of trsv_blocks.hip (narrow
+// right-hand-side tiles, vector FMAs) and trsv_panel.hip (all right-hand
+// sides at once, MFMA): kernel arguments, the system layout of a task, ticket
+// broadcast, the packed layouts of M_B and the gathers that stage r_B or
+// [y_B ; z_R] in LDS.
+#pragma once
+#include "common.hpp"
+#include "launchers.hpp"
+
+namespace cholespy_amd {
+
+template <typename F>
+struct BlockPassArgs {
+    int ntasks, k;
+    int chunk;              // narrow kernels: rows of a vector staged in LDS at a time
+    int lds_bytes;          // panel kernels: dynamic LDS of the launch
+    const int *task;        // (block, first, last) per task
+    const int *start;       // block row ranges
+    const int *rptr;        // R list / partial slot ranges
+    const int *rlist;
+    const int64_t *moff;
+    const F *M;             // forward: column-major packed; backward: row-major packed
+    const int *cptr;        // contribution lists (forward)
+    const int *cslot;
+    const int *perm;
+    const F *b;
+    F *y, *z, *part, *rbuf;
+    void *x;
+    int *status;
+    int xmode;
+    // Several independent systems in one launch: block -> system, and per
+    // system its right-hand-side count and the (possibly negative) bases
+    // that place row r / slot s at base + r * k + t in the packed vectors.
+    // nullptr = one system with a.k columns and zero bases.
+    const int *blk_sys;
+    const int *sys_k;
+    const long long *sys_vbase;
+    const long long *sys_pbase;
+    QueueArgs qa;
+    // Optional task timeline (nullptr = off): per task, 4 words =
+    // {start, after the gather phase, end, workgroup id} in wall-clock ticks (100 MHz).
+    unsigned long long *trace;
+};
+
+// Right-hand-side layout of one task's system.
+struct VecMap {
+    int k;
+    long long vbase, pbase;
+};
+
+// Panel (MFMA) variants of the block passes, trsv_panel.hip.
+int panel_lds_bytes();
+const void *block_panel_kernel(int float_bytes, bool forward);
+hipError_t launch_panel_f32(bool forward, const BlockPassArgs<float> &a, int grid, hipStream_t stream);
+hipError_t launch_panel_f64(bool forward, const BlockPassArgs<double> &a, int grid, hipStream_t stream);
+
+namespace {
+
+constexpr int kTaskThreads = 256;
+constexpr int kGatherUnroll = 8;
+constexpr int kPairs = 4;       // partial-sum gather: (row, column) pairs walked together
+
+template <typename F>
+__device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, int blk) {
+    if (!a.blk_sys) return VecMap{a.k, 0, 0};
+    const int sys = a.blk_sys[blk];
+    return VecMap{a.sys_k[sys], a.sys_vbase[sys], a.sys_pbase[sys]};
+}
+
+// Ticket handling per workgroup: thread 0 takes tickets, the LDS word s_next
+// broadcasts them at task boundaries (after a barrier).
+__device__ __forceinline__ int wg_first_ticket(const QueueArgs &qa, int q, int n, int *s_next) {
+    if (threadIdx.x == 0) *s_next = take_ticket(qa, q, n);
+    __syncthreads();
+    return *s_next;
+}
+
+__device__ __forceinline__ int64_t fwd_off(int64_t rows, int64_t c, int64_t rho) {
+    return c * rows - c * (c - 1) / 2 + (rho - c);
+}
+__device__ __forceinline__ int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {
+    return rho < m ? rho * (rho + 1) / 2 + c : m * (m + 1) / 2 + (rho - m) * m + c;
+}
+
+// Right-hand side r_B for rows [r0, r1) of a block and columns [t0, t0 + kt):
+// (P b) plus the partials other blocks published for those rows. Each thread
+// owns up to kPairs (row, column) pairs and walks their slot lists together,
+// 4 slots per pair per round, so up to 4 * kPairs slot loads are in flight.
+// to_rbuf: publish the sums to rb (read by the block's product tasks);
+// otherwise store them in LDS, row r0 + i at dst[i * stride + t] for
+// t < width (zeros for kt <= t < width).
+template <typename F>
+__device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const F *bv, const F *pv, F *rb,
+                                                F *dst, int a0, int r0, int r1, int t0, int kt, int k,
+                                                int width, int stride, bool to_rbuf) {
+    using U = typename Bits<F>::U;
+    const int total = (r1 - r0) * width;
+    for (int base = threadIdx.x; base < total; base += kTaskThreads * kPairs) {
+        int p0[kPairs], p1[kPairs], col[kPairs];
+        F v[kPairs];
+#pragma unroll
+        for (int q = 0; q < kPairs; ++q) {
+            const int idx = base + q * kTaskThreads;
+            const int i = idx / width, t = idx - i * width;
+            p0[q] = p1[q] = 0;
+            col[q] = t0 + t;
+            v[q] = F(0);
+            if (idx < total && t < kt) {
+                const int row = a0 + r0 + i;
+                v[q] = bv[(long long)a.perm[row] * k + col[q]];
+                p0[q] = a.cptr[row];
+                p1[q] = a.cptr[row + 1];
+            }
+        }
+        for (int r = 0;; r += 4) {
+            bool any = false;
+            U u[kPairs][4];
+            int sl[kPairs][4];
+#pragma unroll
+            for (int q = 0; q < kPairs; ++q)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int p = p0[q] + r + j;
+                    sl[q][j] = p < p1[q] ? a.cslot[p] : -1;
+                    u[q][j] = sl[q][j] >= 0 ? load_bits(&pv[(long long)sl[q][j] * k + col[q]]) : U(0);
+                    any |= sl[q][j] >= 0;
+                }
+            if (!any) break;
+#pragma unroll
+            for (int q = 0; q < kPairs; ++q)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    F x = from_bits<F>(u[q][j]);
+                    if (sl[q][j] >= 0 && u[q][j] == Bits<F>::sentinel)
+                        x = wait_value(&pv[(long long)sl[q][j] * k + col[q]], a.status);
+                    v[q] += x;
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < kPairs; ++q) {
+            const int idx = base + q * kTaskThreads;
+            if (idx >= total) continue;
+            const int i = idx / width, t = idx - i * width;
+            if (to_rbuf) {
+                if (t < kt) publish(&rb[(long long)(a0 + r0 + i) * k + col[q]], v[q]);
+            } else {
+                dst[i * stride + t] = v[q];
+            }
+        }
+    }
+}
+
+// Rows [r0, r1) of a vector into LDS (row r0 + i at dst[i * stride + t] for
+// t < width, zeros for kt <= t < width). `at(row, t, published)` gives the
+// address of an entry and whether it is published by another workgroup
+// (then it is read with an agent-scope load and waited on while it holds
+// the sentinel).
+template <typename F, typename At>
+__device__ __forceinline__ void gather_rows(const BlockPassArgs<F> &a, F *dst, int r0, int r1, int kt,
+                                            int width, int stride, At at) {
+    using U = typename Bits<F>::U;
+    const int total = (r1 - r0) * width;
+    for (int base = threadIdx.x; base < total; base += kTaskThreads * kGatherUnroll) {
+        U u[kGatherUnroll];
+        F *ad[kGatherUnroll];
+        bool pub[kGatherUnroll];
+#pragma unroll
+        for (int j = 0; j < kGatherUnroll; ++j) {
+            const int idx = base + j * kTaskThreads;
+            const int i = idx / width, t = idx - i * width;
+            u[j] = U(0);
+            ad[j] = nullptr;
+            pub[j] = false;
+            if (idx < total && t < kt) {
+                ad[j] = at(r0 + i, t, pub[j]);
+                u[j] = pub[j] ? load_bits(ad[j]) : as_bits(*ad[j]);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < kGatherUnroll; ++j) {
+            const int idx = base + j * kTaskThreads;
+            if (idx >= total) break;
+            F v = from_bits<F>(u[j]);
+            if (pub[j] && u[j] == Bits<F>::sentinel) v = wait_value(ad[j], a.status);
+            const int i = idx / width;
+            dst[i * stride + (idx - i * width)] = v;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace cholespy_amd

@@ -195,8 +195,8 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
             const int b = order[q], m = start[b + 1] - start[b];
             const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
             const int kind = m < opt.split_gather ? 0 : 2;
-            for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows)
-                push(P.fwd_task, b, r0, std::min(rows, r0 + opt.fwd_rows), kind);
+            const int w = kind == 0 ? opt.fwd_rows : opt.fwd_rows_big;
+            for (int r0 = 0; r0 < rows; r0 += w) push(P.fwd_task, b, r0, std::min(rows, r0 + w), kind);
         }
         lo = hi;
     }
@@ -205,7 +205,8 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
                      [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
     for (int b : order) {
         const int m = start[b + 1] - start[b];
-        for (int c0 = 0; c0 < m; c0 += opt.bwd_cols) push(P.bwd_task, b, c0, std::min(m, c0 + opt.bwd_cols), 0);
+        const int w = m < opt.split_gather ? opt.bwd_cols : opt.bwd_cols_big;
+        for (int c0 = 0; c0 < m; c0 += w) push(P.bwd_task, b, c0, std::min(m, c0 + w), 0);
     }
     return P;
 }

@@ -25,6 +25,11 @@ struct BlockPlanOptions {
     int fwd_rows = 64;      // rows of M_B per forward task
     int bwd_cols = 32;      // columns of M_B per backward task
     int split_gather = 65;  // blocks with m >= this gather r_B in separate tasks
+    // Blocks with m >= split_gather are few and sit at the top of the block
+    // tree, where few tasks run at once: their product tasks are narrower so
+    // more workgroups stream each block's M_B concurrently.
+    int fwd_rows_big = 32;  // rows per forward product task of such blocks
+    int bwd_cols_big = 16;  // columns per backward task of such blocks
 };
 
 struct BlockPlan {

@@ -63,7 +63,7 @@ hipError_t upload(T **dst, const T *src, size_t count) {
 
 // Device copy of a BlockPlan (block_plan.hpp) in the solver's precision.
 struct DeviceBlockPlan {
-    int nblocks = 0, nslots = 0, max_rows = 0;
+    int nblocks = 0, nslots = 0, max_m = 0, max_rows = 0;
     int n_fwd_tasks = 0, n_bwd_tasks = 0;
     int levels_fwd = 0, levels_bwd = 0;
     int64_t m_entries = 0;
@@ -159,6 +159,15 @@ BlockPlanOptions plan_options() {
     BlockPlanOptions opt;
     if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
+    // Task widths are limited to what the block kernels' lane layouts cover.
+    auto width = [](const char *name, int &v, int hi) {
+        if (const char *e = std::getenv(name)) v = std::min(hi, std::max(1, std::atoi(e)));
+    };
+    width("CHOLESPY_FWD_ROWS", opt.fwd_rows, 64);
+    width("CHOLESPY_FWD_ROWS_BIG", opt.fwd_rows_big, 64);
+    width("CHOLESPY_BWD_COLS", opt.bwd_cols, 32);
+    width("CHOLESPY_BWD_COLS_BIG", opt.bwd_cols_big, 32);
+    if (const char *e = std::getenv("CHOLESPY_SPLIT_GATHER")) opt.split_gather = std::max(1, std::atoi(e));
     return opt;
 }
 
@@ -168,6 +177,7 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
     d.release();
     d.nblocks = P.nblocks;
     d.nslots = P.rptr[P.nblocks];
+    d.max_m = P.max_m;
     d.max_rows = P.max_rows;
     d.n_fwd_tasks = (int)(P.fwd_task.size() / 4);
     d.n_bwd_tasks = (int)(P.bwd_task.size() / 4);
@@ -539,6 +549,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.max_k = k;
     if (s->is_batch)
         for (int v : s->last_k) bf.max_k = std::max(bf.max_k, v);
+    bf.max_m = d.max_m;
     bf.max_rows = d.max_rows;
     bf.start = d.start;
     bf.rptr = d.rptr;
@@ -571,7 +582,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
         const int tile = block_tile(fbytes, bf.max_k);
         int &grid = s->grid_blocks[forward ? 1 : 0][tile > 4 ? 1 : 0];
         if (grid == 0) {
-            const size_t lds = block_pass_lds(fbytes, d.max_rows, tile);
+            const size_t lds = block_pass_lds(fbytes, forward ? d.max_m : d.max_rows, tile);
             const int cap = max_coresident_blocks(block_pass_kernel(fbytes, forward, tile), 256, lds);
             if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for the block solve");
             grid = std::max(1, std::min(cap, bf.ntasks));

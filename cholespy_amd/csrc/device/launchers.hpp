@@ -50,12 +50,21 @@ const void *row_pass_kernel(int float_bytes, bool lower, int k);
 hipError_t launch_row_pass(int float_bytes, bool lower, const RowPassBuffers &buf, int grid,
                            QueueState &queues, hipStream_t stream);
 
-// Partitioned-inverse block passes (block_plan.hpp / trsv_blocks.hip).
-// Right-hand-side columns per LDS tile: narrow for k <= 4, wide otherwise
-// (fewer passes over M_B per task); the wide tile halves in FP64.
-inline int block_tile(int float_bytes, int k) { return k <= 4 ? 4 : (float_bytes == 4 ? 16 : 8); }
+// Partitioned-inverse block passes (block_plan.hpp / trsv_blocks.hip,
+// trsv_panel.hip). Up to 4 right-hand sides: vector-FMA kernels with one
+// 4-column tile; more: the MFMA panel kernels (all columns in one sweep).
+constexpr int kNarrowTile = 4;
+constexpr int kPanelTile = 16;
+inline int block_tile(int float_bytes, int k) { (void)float_bytes; return k <= kNarrowTile ? kNarrowTile : kPanelTile; }
+// Rows of a vector a narrow block task stages in LDS at a time: a 32 KiB
+// chunk, so four or five workgroups fit a CU's 160 KiB of LDS.
+constexpr int kBlockLdsBytes = 32 * 1024;
+inline int block_chunk(int float_bytes, int tile) { return kBlockLdsBytes / (tile * float_bytes); }
+// Dynamic LDS of a panel launch (override: CHOLESPY_PANEL_LDS, 16-64 KiB).
+constexpr int kPanelLdsBytes = 32 * 1024;
 struct BlockPassBuffers {
-    int ntasks = 0, k = 0, max_rows = 0;
+    int ntasks = 0, k = 0;
+    int max_m = 0, max_rows = 0;   // largest block (m) and M_B height (m + |R|)
     int max_k = 0;                 // largest right-hand-side count of any system (tile choice)
     const int *task = nullptr;     // (block, first, last, kind) per task, in ticket order
     const int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
@@ -75,7 +84,10 @@ struct BlockPassBuffers {
     const int *blk_sys = nullptr, *sys_k = nullptr;
     const long long *sys_vbase = nullptr, *sys_pbase = nullptr;
 };
-size_t block_pass_lds(int float_bytes, int max_rows, int tile);
+// Dynamic LDS of a block pass: one chunk of the vector it stages (r_B of
+// up to need_rows = max m forward, [y_B; z_R] of up to max m + |R| backward),
+// at least 256 rows for the per-thread sums that reuse the space.
+size_t block_pass_lds(int float_bytes, int need_rows, int tile);
 const void *block_pass_kernel(int float_bytes, bool forward, int tile);
 hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffers &buf, int grid,
                              QueueState &queues, hipStream_t stream);

@@ -1,0 +1,397 @@
+// Partitioned-inverse block passes for wide right-hand sides (k > 4) on the
+// gfx950 matrix cores. The tasks, hand-offs and buffers are those of
+// trsv_blocks.hip (block_common.hpp); what changes is the product: a task
+// multiplies its rows (forward) or columns (backward) of M_B by all k
+// right-hand sides in one sweep over M_B, as 16 x 16 output tiles
+// accumulated by v_mfma_{f32,f64}_16x16x4 (A = 16 x 4 piece of M_B from
+// global memory, B = 4 x 16 piece of the staged vector from LDS), instead of
+// one sweep per 4-column tile with vector FMAs.
+//
+// Lane layout of the 16x16x4 MFMAs (checked on gfx950): lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; accumulator element v holds
+// D[4 (l / 16) + v][l % 16] for f32 and D[l / 16 + 4 v][l % 16] for f64.
+//
+// Work split: a task's output is G groups of 16 rows (forward) or 16 columns
+// (backward), G <= 4, times ceil(k / 16) tiles of 16 right-hand sides. The 4
+// wavefronts of the workgroup take groups first, then tiles; when there are
+// fewer (group, tile) pairs than wavefronts, the wavefronts sharing a pair
+// split the 4-wide reduction steps between them and add their partial tiles
+// through LDS at the end.
+//
+// The staged vector (r_B forward, [y_B ; z_R] backward) is held in LDS in
+// chunks of rows with a row stride of ceil(k / 16) * 16 (+16 when that is a
+// multiple of 32) elements, so the four rows one B read touches start in
+// different LDS bank quarters; 4 zero rows follow each chunk because the
+// MFMA steps read B four rows at a time.
+#include "block_common.hpp"
+
+#include <algorithm>
+#include <cstdlib>
+
+namespace cholespy_amd {
+
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename F> struct Mfma;
+template <> struct Mfma<float> {
+    using Acc = f32x4;
+    static __device__ __forceinline__ Acc step(float a, float b, Acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // Output row of accumulator element v in lane group q = lane / 16.
+    static __device__ __forceinline__ int row(int q, int v) { return 4 * q + v; }
+};
+template <> struct Mfma<double> {
+    using Acc = f64x4;
+    static __device__ __forceinline__ Acc step(double a, double b, Acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int q, int v) { return q + 4 * v; }
+};
+
+constexpr int kMaxTiles = kMaxRhs / 16;
+constexpr int kSteps = 4;   // MFMA steps whose A loads are issued together
+
+// Packed offsets of M_B entries (32-bit: one block's M_B has < 2^31 entries).
+__device__ __forceinline__ int fwd_off32(int rows, int c, int rho) {
+    return c * rows - (c * (c - 1)) / 2 + (rho - c);
+}
+__device__ __forceinline__ int bwd_off32(int m, int rho, int c) {
+    return rho < m ? (rho * (rho + 1)) / 2 + c : (m * (m + 1)) / 2 + (rho - m) * m + c;
+}
+
+struct WaveWork {
+    int group;                   // 16-row / 16-column group of this wavefront, -1 = idle
+    int tile0, tstride, ntiles;  // right-hand-side tiles tile0 + j * tstride, j < ntiles
+    int split, nsplit;           // reduction steps s with s % nsplit == split
+    int partner_stride;          // wavefront distance between the splits of one pair
+};
+
+__device__ __forceinline__ WaveWork wave_work(int wave, int width, int nT) {
+    const int G = (width + 15) / 16;
+    const int Gp = G <= 1 ? 1 : (G <= 2 ? 2 : 4);
+    const int W = 4 / Gp;   // wavefronts per group
+    const int nTp = nT <= 1 ? 1 : (nT <= 2 ? 2 : (nT <= 4 ? 4 : 8));
+    WaveWork w;
+    w.group = wave / W;
+    const int u = wave % W;
+    if (nTp >= W) {
+        w.tile0 = u;
+        w.tstride = W;
+        w.ntiles = u < nT ? (nT - u + W - 1) / W : 0;
+        w.split = 0;
+        w.nsplit = 1;
+    } else {
+        w.tile0 = u % nTp;
+        w.tstride = nTp;
+        w.ntiles = w.tile0 < nT ? 1 : 0;
+        w.split = u / nTp;
+        w.nsplit = W / nTp;
+    }
+    w.partner_stride = nTp;
+    if (w.group >= G) {
+        w.group = -1;
+        w.ntiles = 0;
+    }
+    return w;
+}
+
+// LDS row stride of a staged vector with k columns and the rows per chunk.
+__device__ __forceinline__ int panel_stride(int k) {
+    const int kp = (k + 15) / 16 * 16;
+    return kp % 32 == 16 ? kp : kp + 16;
+}
+template <typename F>
+__device__ __forceinline__ int panel_chunk(int lds_bytes, int ks) {
+    return (lds_bytes / (ks * (int)sizeof(F)) - 4) & ~3;
+}
+
+// Zero rows [len, len + 4) of a staged chunk.
+template <typename F>
+__device__ __forceinline__ void zero_tail(F *v, int len, int ks) {
+    for (int idx = threadIdx.x; idx < 4 * ks; idx += kTaskThreads) v[len * ks + idx] = F(0);
+}
+
+// Adds the partial tiles of the other splits of this wavefront's pair (LDS
+// `red`, 64 x 4 values per wavefront). Every thread of the workgroup calls it.
+template <typename F, typename Acc>
+__device__ __forceinline__ void reduce_splits(F *red, const WaveWork &w, int wave, int lane, Acc &acc) {
+    if (w.ntiles > 0 && w.split > 0)
+        for (int v = 0; v < 4; ++v) red[(wave * 64 + lane) * 4 + v] = acc[v];
+    __syncthreads();
+    if (w.ntiles > 0 && w.split == 0)
+        for (int s = 1; s < w.nsplit; ++s) {
+            const int pw = wave + s * w.partner_stride;
+            for (int v = 0; v < 4; ++v) acc[v] += red[(pw * 64 + lane) * 4 + v];
+        }
+}
+
+template <typename F>
+__global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArgs<F> a) {
+    using Acc = typename Mfma<F>::Acc;
+    extern __shared__ __align__(16) unsigned char smem[];
+    F *rv = reinterpret_cast<F *>(smem);
+    __shared__ int s_next;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int qid = blockIdx.x % a.qa.nq;
+    int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
+    while (tk >= 0) {
+        int pre = 0;
+        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
+        if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
+        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
+        const int kind = a.task[4 * tk + 3];
+        const int a0 = a.start[blk];
+        const int m = a.start[blk + 1] - a0;
+        const int rows = m + (a.rptr[blk + 1] - a.rptr[blk]);
+        const F *M = a.M + a.moff[blk];
+        const VecMap vm = vec_map(a, blk);
+        const int k = vm.k;
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
+        const F *bv = a.b + vm.vbase;
+        if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
+            gather_partials(a, bv, pv, rb, (F *)nullptr, a0, first, last, 0, k, k, k, k, true);
+            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+        } else if (k > 0) {
+            const int nT = (k + 15) / 16;
+            const int ks = panel_stride(k);
+            const int chunk = panel_chunk<F>(a.lds_bytes, ks);
+            const WaveWork w = wave_work(wave, last - first, nT);
+            const int g0 = first + 16 * max(w.group, 0);
+            const int row = g0 + (lane & 15);   // A row of this lane
+            const int kk = lane >> 4;           // A column / B row within a step
+            // Columns the group uses: rows < m use c <= row, rows >= m all c < m.
+            const int glast = min(g0 + 16, last) - 1;
+            const int cmax = glast < m ? glast + 1 : m;
+            const int rowlim = row < m ? row + 1 : m;
+            Acc acc[kMaxTiles];
+#pragma unroll
+            for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
+            for (int c0 = 0; c0 < m; c0 += chunk) {
+                const int c1 = min(m, c0 + chunk);
+                if (kind == 0) {
+                    gather_partials(a, bv, pv, rb, rv, a0, c0, c1, 0, k, k, k, ks, false);
+                } else {   // r_B from the gather tasks of this block (published, waited on)
+                    gather_rows(a, rv, c0, c1, k, k, ks, [&](int i, int t, bool &pub) {
+                        pub = true;
+                        return rb + (long long)(a0 + i) * k + t;
+                    });
+                }
+                zero_tail(rv, c1 - c0, ks);
+                __syncthreads();
+                if (a.trace && tid == 0 && c0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+                if (w.ntiles > 0) {
+                    const int lim = min(rowlim, c1);
+                    const int nsteps = (min(c1, cmax) - c0 + 3) / 4;
+                    const F *bl = rv + kk * ks + (lane & 15) + 16 * w.tile0;
+                    for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
+                        F av[kSteps];
+#pragma unroll
+                        for (int q = 0; q < kSteps; ++q) {
+                            const int s = s0 + q * w.nsplit;
+                            const int col = c0 + 4 * s + kk;
+                            av[q] = (s < nsteps && row < last && col < lim) ? M[fwd_off32(rows, col, row)] : F(0);
+                        }
+#pragma unroll
+                        for (int q = 0; q < kSteps; ++q) {
+                            const int s = s0 + q * w.nsplit;
+                            if (s < nsteps) {
+                                const F *bp = bl + 4 * s * ks;
+#pragma unroll
+                                for (int j = 0; j < kMaxTiles; ++j)
+                                    if (j < w.ntiles) acc[j] = Mfma<F>::step(av[q], bp[16 * j * w.tstride], acc[j]);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
+            if (w.ntiles > 0 && w.split == 0) {
+                const long long sbase = a.rptr[blk];
+#pragma unroll
+                for (int j = 0; j < kMaxTiles; ++j) {
+                    if (j >= w.ntiles) break;
+                    const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
+                    if (t >= k) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int rho = g0 + Mfma<F>::row(kk, v);
+                        if (rho >= last) break;
+                        const F s = acc[j][v];
+                        if (rho < m) {
+                            const long long o = (long long)(a0 + rho) * k + t;
+                            yv[o] = s;
+                            zv[o] = from_bits<F>(Bits<F>::sentinel);
+                        } else {
+                            publish(&pv[(sbase + rho - m) * k + t], s);
+                        }
+                    }
+                }
+            }
+        }
+        if (a.trace && tid == 0) {
+            a.trace[4 * (size_t)tk + 2] = wall_clock64();
+            a.trace[4 * (size_t)tk + 3] = blockIdx.x;
+        }
+        if (tid == 0) s_next = pre;
+        __syncthreads();
+        tk = s_next;
+        __syncthreads();
+    }
+    if (tid == 0) finish_queues(a.qa);
+}
+
+template <typename F>
+__global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassArgs<F> a) {
+    using Acc = typename Mfma<F>::Acc;
+    extern __shared__ __align__(16) unsigned char smem[];
+    F *vs = reinterpret_cast<F *>(smem);
+    __shared__ int s_next;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int qid = blockIdx.x % a.qa.nq;
+    int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
+    while (tk >= 0) {
+        int pre = 0;
+        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
+        if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
+        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
+        const int a0 = a.start[blk];
+        const int m = a.start[blk + 1] - a0;
+        const int r0 = a.rptr[blk];
+        const int R = a.rptr[blk + 1] - r0;
+        const int rows = m + R;
+        const F *M = a.M + a.moff[blk];
+        const VecMap vm = vec_map(a, blk);
+        const int k = vm.k;
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+        if (k > 0) {
+            const int nT = (k + 15) / 16;
+            const int ks = panel_stride(k);
+            const int chunk = panel_chunk<F>(a.lds_bytes, ks);
+            const WaveWork w = wave_work(wave, last - first, nT);
+            const int cg0 = first + 16 * max(w.group, 0);
+            const int c = cg0 + (lane & 15);   // A row (= column of M_B) of this lane
+            const int kk = lane >> 4;          // A column / B row within a step
+            Acc acc[kMaxTiles];
+#pragma unroll
+            for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
+            for (int h0 = 0; h0 < rows; h0 += chunk) {
+                const int h1 = min(rows, h0 + chunk);
+                // v = [y_B ; z_R]: y from the forward pass (complete), z waited on.
+                gather_rows(a, vs, h0, h1, k, k, ks, [&](int rho, int t, bool &pub) {
+                    pub = rho >= m;
+                    return pub ? zv + (long long)a.rlist[r0 + rho - m] * k + t : yv + (long long)(a0 + rho) * k + t;
+                });
+                zero_tail(vs, h1 - h0, ks);
+                __syncthreads();
+                if (a.trace && tid == 0 && h0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+                if (w.ntiles > 0) {
+                    // Rows below the group's first column are zero in its columns.
+                    const int hs = max(h0, cg0);
+                    const int nsteps = (h1 - hs + 3) / 4;
+                    const F *bl = vs + (hs - h0 + kk) * ks + (lane & 15) + 16 * w.tile0;
+                    for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
+                        F av[kSteps];
+#pragma unroll
+                        for (int q = 0; q < kSteps; ++q) {
+                            const int s = s0 + q * w.nsplit;
+                            const int rho = hs + 4 * s + kk;
+                            av[q] = (s < nsteps && c < last && rho < h1 && rho >= c) ? M[bwd_off32(m, rho, c)] : F(0);
+                        }
+#pragma unroll
+                        for (int q = 0; q < kSteps; ++q) {
+                            const int s = s0 + q * w.nsplit;
+                            if (s < nsteps) {
+                                const F *bp = bl + 4 * s * ks;
+#pragma unroll
+                                for (int j = 0; j < kMaxTiles; ++j)
+                                    if (j < w.ntiles) acc[j] = Mfma<F>::step(av[q], bp[16 * j * w.tstride], acc[j]);
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+            if (w.nsplit > 1) reduce_splits(vs, w, wave, lane, acc[0]);
+            if (w.ntiles > 0 && w.split == 0) {
+#pragma unroll
+                for (int j = 0; j < kMaxTiles; ++j) {
+                    if (j >= w.ntiles) break;
+                    const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
+                    if (t >= k) continue;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int cc = cg0 + Mfma<F>::row(kk, v);
+                        if (cc >= last) break;
+                        const F s = acc[j][v];
+                        publish(&zv[(long long)(a0 + cc) * k + t], s);
+                        const long long o = vm.vbase + (long long)a.perm[a0 + cc] * k + t;
+                        if (a.xmode == 0) {
+                            static_cast<F *>(a.x)[o] = s;
+                        } else {
+                            double *x64 = static_cast<double *>(a.x);
+                            x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
+                        }
+                    }
+                }
+            }
+        }
+        // Restore the sentinel in this block's partial slots for the next solve.
+        if (first == 0) {
+            const F sent = from_bits<F>(Bits<F>::sentinel);
+            for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
+            F *rb = a.rbuf + vm.vbase;
+            for (int idx = tid; idx < m * k; idx += kTaskThreads) rb[(long long)a0 * k + idx] = sent;
+        }
+        if (a.trace && tid == 0) {
+            a.trace[4 * (size_t)tk + 2] = wall_clock64();
+            a.trace[4 * (size_t)tk + 3] = blockIdx.x;
+        }
+        if (tid == 0) s_next = pre;
+        __syncthreads();
+        tk = s_next;
+        __syncthreads();
+    }
+    if (tid == 0) finish_queues(a.qa);
+}
+
+}  // namespace
+
+int panel_lds_bytes() {
+    static const int bytes = [] {
+        int v = kPanelLdsBytes;
+        if (const char *e = std::getenv("CHOLESPY_PANEL_LDS")) v = std::atoi(e);
+        return std::min(std::max(v, 16 * 1024), 64 * 1024);
+    }();
+    return bytes;
+}
+
+const void *block_panel_kernel(int float_bytes, bool forward) {
+    if (float_bytes == 4)
+        return forward ? reinterpret_cast<const void *>(&block_forward_panel<float>)
+                       : reinterpret_cast<const void *>(&block_backward_panel<float>);
+    return forward ? reinterpret_cast<const void *>(&block_forward_panel<double>)
+                   : reinterpret_cast<const void *>(&block_backward_panel<double>);
+}
+
+template <typename F>
+static hipError_t launch_panel_impl(bool forward, const BlockPassArgs<F> &args, int grid, hipStream_t stream) {
+    BlockPassArgs<F> a = args;
+    a.lds_bytes = panel_lds_bytes();
+    void *p[] = {&a};
+    return hipLaunchKernel(block_panel_kernel(sizeof(F), forward), dim3(grid), dim3(kTaskThreads), p,
+                           (size_t)a.lds_bytes, stream);
+}
+
+hipError_t launch_panel_f32(bool forward, const BlockPassArgs<float> &a, int grid, hipStream_t stream) {
+    return launch_panel_impl<float>(forward, a, grid, stream);
+}
+hipError_t launch_panel_f64(bool forward, const BlockPassArgs<double> &a, int grid, hipStream_t stream) {
+    return launch_panel_impl<double>(forward, a, grid, stream);
+}
+
+}  // namespace cholespy_amd

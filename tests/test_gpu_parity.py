@@ -59,7 +59,7 @@ def test_solve_matches_oracle(case, cls, dtype, path):
     solver._set_path(PATHS[path])
     fac = solver._factor()
     rng = np.random.default_rng(1)
-    for k in (1, 3, 4, 16, 64, 65, 128):
+    for k in (1, 3, 4, 5, 16, 17, 33, 64, 65, 128):
         b = rng.random((n, k)).astype(dtype)
         bt = torch.tensor(b, device=DEV)
         xt = torch.zeros_like(bt)
