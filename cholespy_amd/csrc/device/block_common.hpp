@@ -4,6 +4,7 @@
 // broadcast, the packed layouts of M_B and the gathers that stage r_B or
 // [y_B ; z_R] in LDS.
 #pragma once
+#include "block_plan.hpp"
 #include "common.hpp"
 #include "launchers.hpp"
 
@@ -14,7 +15,7 @@ struct BlockPassArgs {
     int ntasks, k;
     int chunk;              // narrow kernels: rows of a vector staged in LDS at a time
     int lds_bytes;          // panel kernels: dynamic LDS of the launch
-    const int *task;        // (block, first, last) per task
+    const int *task;        // kTaskInts ints per task (block_plan.hpp)
     const int *start;       // block row ranges
     const int *rptr;        // R list / partial slot ranges
     const int *rlist;
@@ -25,6 +26,7 @@ struct BlockPassArgs {
     const int *perm;
     const F *b;
     F *y, *z, *part, *rbuf;
+    F *spart;               // split results (split rows x k)
     void *x;
     int *status;
     int xmode;
@@ -36,6 +38,7 @@ struct BlockPassArgs {
     const int *sys_k;
     const long long *sys_vbase;
     const long long *sys_pbase;
+    const long long *sys_sbase;
     QueueArgs qa;
     // Optional task timeline (nullptr = off): per task, 4 words =
     // {start, after the gather phase, end, workgroup id} in wall-clock ticks (100 MHz).
@@ -45,7 +48,12 @@ struct BlockPassArgs {
 // Right-hand-side layout of one task's system.
 struct VecMap {
     int k;
-    long long vbase, pbase;
+    long long vbase, pbase, sbase;
+};
+
+// One task record (block_plan.hpp).
+struct Task {
+    int blk, first, last, kind, lo, hi, sbase, nsplit;
 };
 
 // Panel (MFMA) variants of the block passes, trsv_panel.hip.
@@ -62,9 +70,51 @@ constexpr int kPairs = 4;       // partial-sum gather: (row, column) pairs walke
 
 template <typename F>
 __device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, int blk) {
-    if (!a.blk_sys) return VecMap{a.k, 0, 0};
+    if (!a.blk_sys) return VecMap{a.k, 0, 0, 0};
     const int sys = a.blk_sys[blk];
-    return VecMap{a.sys_k[sys], a.sys_vbase[sys], a.sys_pbase[sys]};
+    return VecMap{a.sys_k[sys], a.sys_vbase[sys], a.sys_pbase[sys], a.sys_sbase[sys]};
+}
+
+template <typename F>
+__device__ __forceinline__ Task load_task(const BlockPassArgs<F> &a, int tk) {
+    const int *t = a.task + (size_t)kTaskInts * tk;
+    return Task{t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7]};
+}
+
+// Value of a published entry: one agent-scope load, and the bounded wait of
+// common.hpp only when it still holds the sentinel.
+template <typename F>
+__device__ __forceinline__ F ready_value(const F *p, typename Bits<F>::U u, int *status) {
+    return u == Bits<F>::sentinel ? wait_value(p, status) : from_bits<F>(u);
+}
+
+// Combine task: the nsplit split results of w rows (split s, row i at split
+// row sbase + s * w + i) added in split order, so the sum does not depend on
+// timing; each split value is reset to the sentinel after it is read (the
+// combine is its only reader). out(i, t, value) finishes row i, column t.
+constexpr int kMaxSplit = 16;
+template <typename F, typename Out>
+__device__ __forceinline__ void combine_splits(const BlockPassArgs<F> &a, F *sp, int w, int k, int sbase, int nsplit,
+                                               Out out) {
+    using U = typename Bits<F>::U;
+    const int total = w * k;
+    for (int idx = threadIdx.x; idx < total; idx += kTaskThreads) {
+        const int i = idx / k, t = idx - i * k;
+        F *p0 = sp + (long long)(sbase + i) * k + t;
+        const long long stride = (long long)w * k;
+        U u[kMaxSplit];
+#pragma unroll
+        for (int s = 0; s < kMaxSplit; ++s) u[s] = s < nsplit ? load_bits(p0 + s * stride) : U(0);
+        F v = F(0);
+#pragma unroll
+        for (int s = 0; s < kMaxSplit; ++s)
+            if (s < nsplit) v += ready_value(p0 + s * stride, u[s], a.status);
+        const F sent = from_bits<F>(Bits<F>::sentinel);
+#pragma unroll
+        for (int s = 0; s < kMaxSplit; ++s)
+            if (s < nsplit) p0[s * stride] = sent;
+        out(i, t, v);
+    }
 }
 
 // Ticket handling per workgroup: thread 0 takes tickets, the LDS word s_next

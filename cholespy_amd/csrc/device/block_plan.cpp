@@ -1,5 +1,7 @@
 #include "block_plan.hpp"
 
+#include <functional>
+
 #include <algorithm>
 #include <numeric>
 #include <stdexcept>
@@ -170,20 +172,38 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     }
 
     // 5. Tasks in level order (stable in block index).
+    int next_split = 0;   // split rows: their own array, numbered per plan
+    auto push = [](std::vector<int> &v, int b, int f, int l, int kind, int lo = 0, int hi = 0, int sb = 0,
+                   int ns = 0) {
+        const int rec[kTaskInts] = {b, f, l, kind, lo, hi, sb, ns};
+        v.insert(v.end(), rec, rec + kTaskInts);
+    };
+    // Cuts [lo, hi) into at most `parts` ranges of about equal weight.
+    auto cut = [](int lo, int hi, int parts, const std::function<int64_t(int)> &w) {
+        int64_t total = 0;
+        for (int x = lo; x < hi; ++x) total += w(x);
+        std::vector<int> b{lo};
+        int64_t acc = 0;
+        for (int x = lo; x < hi; ++x) {
+            acc += w(x);
+            if ((int)b.size() < parts && acc * parts >= total * (int64_t)b.size() && x + 1 < hi) b.push_back(x + 1);
+        }
+        b.push_back(hi);
+        return b;
+    };
+    auto nsplits = [&](int64_t entries) {
+        if (opt.split_entries <= 0 || entries <= opt.split_entries) return 1;
+        return (int)std::min<int64_t>(opt.max_split, (entries + opt.split_entries - 1) / opt.split_entries);
+    };
     std::vector<int> order(nb);
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
-    auto push = [](std::vector<int> &v, int b, int f, int l, int kind) {
-        v.push_back(b);
-        v.push_back(f);
-        v.push_back(l);
-        v.push_back(kind);
-    };
+    std::vector<int> combines;
     for (size_t lo = 0; lo < order.size();) {
         // One level at a time: the gather tasks of its large blocks first,
-        // then the product tasks, so a product task's r_B is gathered by
-        // tasks with smaller tickets.
+        // then the product tasks, then the combine tasks, so every task's
+        // inputs come from tasks with smaller tickets.
         size_t hi = lo;
         while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
         for (size_t q = lo; q < hi; ++q) {
@@ -191,23 +211,81 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
             if (m < opt.split_gather) continue;
             for (int r0 = 0; r0 < m; r0 += opt.fwd_rows) push(P.fwd_task, b, r0, std::min(m, r0 + opt.fwd_rows), 1);
         }
+        combines.clear();
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q], m = start[b + 1] - start[b];
             const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
-            const int kind = m < opt.split_gather ? 0 : 2;
-            const int w = kind == 0 ? opt.fwd_rows : opt.fwd_rows_big;
-            for (int r0 = 0; r0 < rows; r0 += w) push(P.fwd_task, b, r0, std::min(rows, r0 + w), kind);
+            if (m < opt.split_gather) {
+                for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows)
+                    push(P.fwd_task, b, r0, std::min(rows, r0 + opt.fwd_rows), 0);
+                continue;
+            }
+            for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows_big) {
+                const int r1 = std::min(rows, r0 + opt.fwd_rows_big), w = r1 - r0;
+                // Column c feeds the task rows rho >= c (rho < m) and all rows >= m.
+                auto weight = [&](int c) -> int64_t { return r1 - std::max(r0, std::min(c, m)); };
+                const int cuse = std::min(r1, m);
+                int64_t entries = 0;
+                for (int c = 0; c < cuse; ++c) entries += weight(c);
+                const int ns = nsplits(entries);
+                if (ns == 1) {
+                    push(P.fwd_task, b, r0, r1, 2);
+                    continue;
+                }
+                const std::vector<int> cb = cut(0, cuse, ns, weight);
+                const int parts = (int)cb.size() - 1;
+                const int base = next_split;
+                for (int s2 = 0; s2 < parts; ++s2)
+                    push(P.fwd_task, b, r0, r1, 3, cb[s2], cb[s2 + 1], base + s2 * w, parts);
+                next_split += parts * w;
+                const int rec[kTaskInts] = {b, r0, r1, 4, 0, 0, base, parts};
+                combines.insert(combines.end(), rec, rec + kTaskInts);
+            }
         }
+        P.fwd_task.insert(P.fwd_task.end(), combines.begin(), combines.end());
         lo = hi;
     }
+    // The backward pass reuses the split rows: each combine resets its rows
+    // before its kernel ends.
+    const int fwd_split_rows = next_split;
+    next_split = 0;
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
-    for (int b : order) {
-        const int m = start[b + 1] - start[b];
-        const int w = m < opt.split_gather ? opt.bwd_cols : opt.bwd_cols_big;
-        for (int c0 = 0; c0 < m; c0 += w) push(P.bwd_task, b, c0, std::min(m, c0 + w), 0);
+    for (size_t lo = 0; lo < order.size();) {
+        size_t hi = lo;
+        while (hi < order.size() && P.level_bwd[order[hi]] == P.level_bwd[order[lo]]) ++hi;
+        combines.clear();
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q], m = start[b + 1] - start[b];
+            const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
+            const bool big = m >= opt.split_gather;
+            const int wmax = big ? opt.bwd_cols_big : opt.bwd_cols;
+            for (int c0 = 0; c0 < m; c0 += wmax) {
+                const int c1 = std::min(m, c0 + wmax), w = c1 - c0;
+                // Row rho feeds the task columns c <= rho.
+                auto weight = [&](int rho) -> int64_t { return std::min(rho + 1, c1) - c0; };
+                int64_t entries = 0;
+                for (int rho = c0; rho < rows; ++rho) entries += weight(rho);
+                const int ns = big ? nsplits(entries) : 1;
+                if (ns == 1) {
+                    push(P.bwd_task, b, c0, c1, 0);
+                    continue;
+                }
+                const std::vector<int> hb = cut(c0, rows, ns, weight);
+                const int parts = (int)hb.size() - 1;
+                const int base = next_split;
+                for (int s2 = 0; s2 < parts; ++s2)
+                    push(P.bwd_task, b, c0, c1, 1, hb[s2], hb[s2 + 1], base + s2 * w, parts);
+                next_split += parts * w;
+                const int rec[kTaskInts] = {b, c0, c1, 2, 0, 0, base, parts};
+                combines.insert(combines.end(), rec, rec + kTaskInts);
+            }
+        }
+        P.bwd_task.insert(P.bwd_task.end(), combines.begin(), combines.end());
+        lo = hi;
     }
+    P.nsplit_rows = std::max(fwd_split_rows, next_split);
     return P;
 }
 

@@ -19,6 +19,8 @@
 
 namespace cholespy_amd {
 
+constexpr int kTaskInts = 8;   // ints per task record (CHOLESPY_TASK_INTS of the C ABI)
+
 struct BlockPlanOptions {
     int max_block = 1024;   // rows per block (dense inverse is m x m)
     int leaf_max = 32;      // whole subtrees up to this size become one block
@@ -30,6 +32,12 @@ struct BlockPlanOptions {
     // more workgroups stream each block's M_B concurrently.
     int fwd_rows_big = 32;  // rows per forward product task of such blocks
     int bwd_cols_big = 16;  // columns per backward task of such blocks
+    // A product task of such a block that covers more than split_entries
+    // entries of M_B is split along its reduction (columns forward, rows
+    // backward) into up to max_split tasks of about split_entries entries
+    // whose partial results a combine task adds (0 = no splitting).
+    int split_entries = 4096;
+    int max_split = 16;
 };
 
 struct BlockPlan {
@@ -48,12 +56,20 @@ struct BlockPlan {
     std::vector<double> mfwd, mbwd;
     // Block level in the block tree: forward level (height) and backward level (depth).
     std::vector<int> level_fwd, level_bwd;
-    // Tasks in schedule order, 4 ints each: (block, first, last, kind).
+    // Tasks in schedule order, kTaskInts ints each:
+    //   (block, first, last, kind, lo, hi, sbase, nsplit).
     // Forward kinds: 0 = gather r_B and multiply rows [first, last) of M_B;
-    // 1 = gather rows [first, last) of r_B only (published for kind 2);
-    // 2 = multiply rows [first, last) of M_B by the published r_B.
-    // Backward (kind 0): columns [first, last) of M_B.
+    // 1 = gather rows [first, last) of r_B only (published for kinds 2, 3);
+    // 2 = multiply rows [first, last) of M_B by the published r_B;
+    // 3 = as 2 over columns [lo, hi) only, result published to the split
+    //     rows sbase + (rho - first) (an array of its own, k values a row);
+    // 4 = combine: add the nsplit split results of rows [first, last)
+    //     (split s at rows sbase + s * (last - first)) and finish as kind 2.
+    // Backward kinds: 0 = columns [first, last) of M_B; 1 = as 0 over rows
+    // [lo, hi) only, published to split rows sbase + (c - first); 2 =
+    // combine nsplit split results (as forward kind 4) and finish as kind 0.
     std::vector<int> fwd_task, bwd_task;
+    int nsplit_rows = 0;                   // rows of the split-result array
     int max_m = 0, max_rows = 0;           // largest m and m + |R|
     int64_t nnz_l = 0;
 };

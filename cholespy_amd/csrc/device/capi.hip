@@ -63,7 +63,7 @@ hipError_t upload(T **dst, const T *src, size_t count) {
 
 // Device copy of a BlockPlan (block_plan.hpp) in the solver's precision.
 struct DeviceBlockPlan {
-    int nblocks = 0, nslots = 0, max_m = 0, max_rows = 0;
+    int nblocks = 0, nslots = 0, nsplit_rows = 0, max_m = 0, max_rows = 0;
     int n_fwd_tasks = 0, n_bwd_tasks = 0;
     int levels_fwd = 0, levels_bwd = 0;
     int64_t m_entries = 0;
@@ -115,6 +115,8 @@ struct cholespy_gpu_solver {
     int *status = nullptr;                // [0] error code, [1] abort flag
     void *y = nullptr, *z = nullptr;      // per-solve work buffers (n x k)
     void *part = nullptr;                 // block path partial slots (nslots x k)
+    void *spart = nullptr;                // block path split results (nsplit_rows x k)
+    size_t spart_elems = 0;
     void *rbuf = nullptr;                 // block path gathered right-hand sides (n x k)
     int k_cur = 0;                        // layout the work buffers hold
     int path = CHOLESPY_PATH_BLOCKS;
@@ -135,19 +137,20 @@ struct cholespy_gpu_solver {
     unsigned int *qctr = nullptr;
     QueueState queues[6];
     unsigned long long *trace[2] = {nullptr, nullptr};   // block pass timelines [backward, forward]
+    std::vector<int> plan_copy[6];        // host copies of the plan for cholespy_solver_plan_arrays
     // Batches of independent systems (cholespy_batch_*): rows are
     // concatenated; per system its size, partial-slot count and host factor
     // until finalize; per call the RHS counts and vector bases.
     int nsys = 1;
     bool is_batch = false;
-    std::vector<int> sys_n, sys_row0, sys_slots, sys_slot0;
+    std::vector<int> sys_n, sys_row0, sys_slots, sys_slot0, sys_splits, sys_split0;
     struct HostFactor {
         std::vector<int> perm, colptr, rowidx;
         std::vector<double> val;
     };
     std::vector<HostFactor> pending;
     int *blk_sys = nullptr, *sys_k = nullptr;
-    long long *sys_vbase = nullptr, *sys_pbase = nullptr;
+    long long *sys_vbase = nullptr, *sys_pbase = nullptr, *sys_sbase = nullptr;
     std::vector<int> last_k;   // RHS counts the per-system arrays hold
     std::vector<int> host_levels[2], host_row_level[2];
 };
@@ -168,6 +171,8 @@ BlockPlanOptions plan_options() {
     width("CHOLESPY_BWD_COLS", opt.bwd_cols, 32);
     width("CHOLESPY_BWD_COLS_BIG", opt.bwd_cols_big, 32);
     if (const char *e = std::getenv("CHOLESPY_SPLIT_GATHER")) opt.split_gather = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_SPLIT_ENTRIES")) opt.split_entries = std::max(0, std::atoi(e));
+    width("CHOLESPY_MAX_SPLIT", opt.max_split, 16);
     return opt;
 }
 
@@ -177,14 +182,21 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
     d.release();
     d.nblocks = P.nblocks;
     d.nslots = P.rptr[P.nblocks];
+    d.nsplit_rows = P.nsplit_rows;
     d.max_m = P.max_m;
     d.max_rows = P.max_rows;
-    d.n_fwd_tasks = (int)(P.fwd_task.size() / 4);
-    d.n_bwd_tasks = (int)(P.bwd_task.size() / 4);
+    d.n_fwd_tasks = (int)(P.fwd_task.size() / kTaskInts);
+    d.n_bwd_tasks = (int)(P.bwd_task.size() / kTaskInts);
     d.levels_fwd = d.levels_bwd = 0;
     for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
     for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
     d.m_entries = P.moff[P.nblocks];
+    s->plan_copy[0] = P.start;
+    s->plan_copy[1] = P.rptr;
+    s->plan_copy[2] = P.level_fwd;
+    s->plan_copy[3] = P.level_bwd;
+    s->plan_copy[4] = P.fwd_task;
+    s->plan_copy[5] = P.bwd_task;
     HIP_TRY(upload(&d.start, P.start.data(), P.start.size()));
     HIP_TRY(upload(&d.rptr, P.rptr.data(), P.rptr.size()));
     HIP_TRY(upload(&d.rlist, P.rlist.data(), P.rlist.size()));
@@ -299,11 +311,17 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->y);
     dev_free(s->z);
     dev_free(s->part);
+    dev_free(s->spart);
     dev_free(s->rbuf);
     dev_free(s->arowptr);
     dev_free(s->acols);
     dev_free(s->avals);
     dev_free(s->r32);
+    dev_free(s->blk_sys);
+    dev_free(s->sys_k);
+    dev_free(s->sys_vbase);
+    dev_free(s->sys_pbase);
+    dev_free(s->sys_sbase);
     s->plan.release();
     delete s;
     return CHOLESPY_OK;
@@ -465,6 +483,8 @@ int cholespy_solver_check(cholespy_gpu_solver *s) {
             HIP_TRY(launch_fill_sentinel(s->z, s->work_elems, s->fbytes, nullptr));
             if (s->part && s->part_elems)
                 HIP_TRY(launch_fill_sentinel(s->part, s->part_elems, s->fbytes, nullptr));
+            if (s->spart && s->spart_elems)
+                HIP_TRY(launch_fill_sentinel(s->spart, s->spart_elems, s->fbytes, nullptr));
             if (s->rbuf) HIP_TRY(launch_fill_sentinel(s->rbuf, s->work_elems, s->fbytes, nullptr));
             HIP_TRY(hipDeviceSynchronize());
         }
@@ -499,12 +519,20 @@ int prepare_buffers(cholespy_gpu_solver *s, int k, hipStream_t st) {
         s->part_elems = pe;
         s->k_cur = 0;
     }
+    const size_t se = (size_t)s->plan.nsplit_rows * (size_t)k;
+    if (s->path == CHOLESPY_PATH_BLOCKS && se > s->spart_elems) {
+        dev_free(s->spart);
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->spart), std::max<size_t>(se, 1) * (size_t)fbytes));
+        s->spart_elems = se;
+        s->k_cur = 0;
+    }
     if (k != s->k_cur || s->buffers_path != s->path) {
         // Buffers are indexed row * k + col: refill them for the new layout.
         HIP_TRY(launch_fill_sentinel(s->y, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->z, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->rbuf, elems, fbytes, st));
         if (s->part && pe) HIP_TRY(launch_fill_sentinel(s->part, pe, fbytes, st));
+        if (s->spart && se) HIP_TRY(launch_fill_sentinel(s->spart, se, fbytes, st));
         s->k_cur = k;
         s->buffers_path = s->path;
     }
@@ -563,6 +591,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.y = s->y;
     bf.z = s->z;
     bf.part = s->part;
+    bf.spart = s->spart;
     bf.rbuf = s->rbuf;
     bf.status = s->status;
     bf.xmode = xmode;
@@ -571,6 +600,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
         bf.sys_k = s->sys_k;
         bf.sys_vbase = s->sys_vbase;
         bf.sys_pbase = s->sys_pbase;
+        bf.sys_sbase = s->sys_sbase;
     }
     for (int pass = 0; pass < 2; ++pass) {
         if (only_pass >= 0 && pass != only_pass) continue;
@@ -818,7 +848,9 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     std::vector<int> blk_sys, gperm((size_t)s->n);
     s->sys_slots.assign((size_t)ns, 0);
     s->sys_slot0.assign((size_t)ns, 0);
-    int B0 = 0, S0 = 0, C0 = 0;
+    s->sys_splits.assign((size_t)ns, 0);
+    s->sys_split0.assign((size_t)ns, 0);
+    int B0 = 0, S0 = 0, C0 = 0, SP0 = 0;
     int64_t M0 = 0;
     struct TaskRef { int level, sys, idx; };
     std::vector<TaskRef> ftasks, btasks;
@@ -827,6 +859,8 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         const int R0 = s->sys_row0[i];
         s->sys_slot0[i] = S0;
         s->sys_slots[i] = P.rptr[P.nblocks];
+        s->sys_split0[i] = SP0;
+        s->sys_splits[i] = P.nsplit_rows;
         for (int r = 0; r < s->sys_n[i]; ++r) gperm[(size_t)R0 + r] = R0 + s->pending[i].perm[r];
         for (int b = 0; b < P.nblocks; ++b) {
             G.start.push_back(P.start[b] + R0);
@@ -841,16 +875,17 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         G.mbwd.insert(G.mbwd.end(), P.mbwd.begin(), P.mbwd.end());
         for (int r = 0; r < s->sys_n[i]; ++r) G.contrib_ptr.push_back(P.contrib_ptr[r] + C0);
         for (int v : P.contrib_slot) G.contrib_slot.push_back(v + S0);
-        for (size_t t = 0; t < P.fwd_task.size() / 4; ++t)
-            ftasks.push_back({P.level_fwd[P.fwd_task[4 * t]], i, (int)t});
-        for (size_t t = 0; t < P.bwd_task.size() / 4; ++t)
-            btasks.push_back({P.level_bwd[P.bwd_task[4 * t]], i, (int)t});
+        for (size_t t = 0; t < P.fwd_task.size() / kTaskInts; ++t)
+            ftasks.push_back({P.level_fwd[P.fwd_task[kTaskInts * t]], i, (int)t});
+        for (size_t t = 0; t < P.bwd_task.size() / kTaskInts; ++t)
+            btasks.push_back({P.level_bwd[P.bwd_task[kTaskInts * t]], i, (int)t});
         G.max_m = std::max(G.max_m, P.max_m);
         G.max_rows = std::max(G.max_rows, P.max_rows);
         G.nnz_l += P.nnz_l;
         s->sys_row0[i] = R0;
         B0 += P.nblocks;
         S0 += P.rptr[P.nblocks];
+        SP0 += P.nsplit_rows;
         C0 += P.contrib_ptr[s->sys_n[i]];
         M0 += P.moff[P.nblocks];
     }
@@ -859,16 +894,18 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     G.rptr.push_back(S0);
     G.moff.push_back(M0);
     G.contrib_ptr.push_back(C0);
+    G.nsplit_rows = SP0;
     std::vector<int> blk0((size_t)ns + 1, 0);
     for (int i = 0; i < ns; ++i) blk0[i + 1] = blk0[i] + plans[i].nblocks;
     auto emit = [&](std::vector<TaskRef> &refs, bool fwd, std::vector<int> &out) {
         std::stable_sort(refs.begin(), refs.end(), [](const TaskRef &x, const TaskRef &y) { return x.level < y.level; });
         for (const TaskRef &r : refs) {
             const std::vector<int> &src = fwd ? plans[r.sys].fwd_task : plans[r.sys].bwd_task;
-            out.push_back(src[4 * r.idx] + blk0[r.sys]);
-            out.push_back(src[4 * r.idx + 1]);
-            out.push_back(src[4 * r.idx + 2]);
-            out.push_back(src[4 * r.idx + 3]);
+            const int *t = &src[(size_t)kTaskInts * r.idx];
+            // block and split-row base move by the system's offsets
+            const int rec[kTaskInts] = {t[0] + blk0[r.sys], t[1], t[2], t[3], t[4], t[5],
+                                        t[7] > 0 ? t[6] + s->sys_split0[r.sys] : 0, t[7]};
+            out.insert(out.end(), rec, rec + kTaskInts);
         }
     };
     emit(ftasks, true, G.fwd_task);
@@ -882,9 +919,11 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     dev_free(s->sys_k);
     dev_free(s->sys_vbase);
     dev_free(s->sys_pbase);
+    dev_free(s->sys_sbase);
     HIP_TRY(dev_alloc(&s->sys_k, (size_t)ns));
     HIP_TRY(dev_alloc(&s->sys_vbase, (size_t)ns));
     HIP_TRY(dev_alloc(&s->sys_pbase, (size_t)ns));
+    HIP_TRY(dev_alloc(&s->sys_sbase, (size_t)ns));
     s->last_k.clear();
     s->pending.clear();
     s->uploaded = true;
@@ -902,28 +941,32 @@ int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void
     if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "batch not finalised");
     const int ns = s->nsys;
     std::vector<int> ks(n_rhs, n_rhs + ns);
-    size_t elems = 0, pe = 0;
+    size_t elems = 0, pe = 0, se = 0;
     for (int i = 0; i < ns; ++i) {
         if (ks[i] < 0 || ks[i] > kMaxRhs)
             return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
         elems += (size_t)s->sys_n[i] * ks[i];
         pe += (size_t)s->sys_slots[i] * ks[i];
+        se += (size_t)s->sys_splits[i] * ks[i];
     }
     if (elems == 0) return CHOLESPY_OK;
     if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (ks != s->last_k) {
-        std::vector<long long> vb((size_t)ns), pb((size_t)ns);
-        long long voff = 0, poff = 0;
+        std::vector<long long> vb((size_t)ns), pb((size_t)ns), sb((size_t)ns);
+        long long voff = 0, poff = 0, soff = 0;
         for (int i = 0; i < ns; ++i) {
             vb[i] = voff - (long long)s->sys_row0[i] * ks[i];
             pb[i] = poff - (long long)s->sys_slot0[i] * ks[i];
+            sb[i] = soff - (long long)s->sys_split0[i] * ks[i];
             voff += (long long)s->sys_n[i] * ks[i];
             poff += (long long)s->sys_slots[i] * ks[i];
+            soff += (long long)s->sys_splits[i] * ks[i];
         }
         HIP_TRY(hipMemcpy(s->sys_k, ks.data(), (size_t)ns * sizeof(int), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s->sys_vbase, vb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s->sys_pbase, pb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s->sys_sbase, sb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
         if (elems > s->work_elems) {
             dev_free(s->y);
             dev_free(s->z);
@@ -938,10 +981,16 @@ int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->part), pe * (size_t)fbytes));
             s->part_elems = pe;
         }
+        if (se > s->spart_elems) {
+            dev_free(s->spart);
+            HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->spart), se * (size_t)fbytes));
+            s->spart_elems = se;
+        }
         HIP_TRY(launch_fill_sentinel(s->y, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->z, elems, fbytes, st));
         HIP_TRY(launch_fill_sentinel(s->rbuf, elems, fbytes, st));
         if (pe) HIP_TRY(launch_fill_sentinel(s->part, pe, fbytes, st));
+        if (se) HIP_TRY(launch_fill_sentinel(s->spart, se, fbytes, st));
         s->last_k = ks;
         s->buffers_path = CHOLESPY_PATH_BLOCKS;
     }
@@ -973,6 +1022,16 @@ int cholespy_solver_plan_info(const cholespy_gpu_solver *s, int64_t *info, int c
     const int64_t vals[] = {d.nblocks, d.m_entries, d.levels_fwd, d.levels_bwd,
                             d.n_fwd_tasks, d.n_bwd_tasks, d.nslots, d.max_rows};
     for (int i = 0; i < count && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_plan_arrays(const cholespy_gpu_solver *s, int which, int32_t *out, int64_t count,
+                                int64_t *size) {
+    if (!s || which < 0 || which > 5 || (count > 0 && !out)) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "bad argument");
+    const std::vector<int> &v = s->plan_copy[which];
+    if (size) *size = (int64_t)v.size();
+    const int64_t nc = std::min<int64_t>(count, (int64_t)v.size());
+    for (int64_t i = 0; i < nc; ++i) out[i] = v[(size_t)i];
     return CHOLESPY_OK;
 }
 

@@ -76,6 +76,9 @@ __device__ __forceinline__ void raise_timeout(int *status) {
 }
 
 constexpr uint32_t kSpinLimit = 1u << 23;
+#ifndef CHOLESPY_SPIN_SLEEP
+#define CHOLESPY_SPIN_SLEEP 1
+#endif
 
 // Waits until *p is no longer the sentinel and returns the value. On
 // timeout or abort returns 0 (the kernel's result is then invalid and the
@@ -87,7 +90,7 @@ __device__ __forceinline__ F wait_value(const F *p, int *status) {
     if (u != Bits<F>::sentinel) return from_bits<F>(u);
     uint32_t spins = 0;
     while (true) {
-        __builtin_amdgcn_s_sleep(1);
+        __builtin_amdgcn_s_sleep(CHOLESPY_SPIN_SLEEP);
         u = load_bits(p);
         if (u != Bits<F>::sentinel) return from_bits<F>(u);
         if ((++spins & 255u) == 0) {

@@ -74,6 +74,7 @@ struct BlockPassBuffers {
     const int *perm = nullptr;
     const void *b = nullptr;
     void *y = nullptr, *z = nullptr, *part = nullptr, *x = nullptr;
+    void *spart = nullptr;         // split results of split tasks (split rows x k, sentinel-protected)
     void *rbuf = nullptr;          // gathered r_B of large blocks (n x k, sentinel-protected)
     int *status = nullptr;
     // Backward epilogue: 0 = x (solver precision) = z; 1 = x (double) = z;
@@ -82,7 +83,7 @@ struct BlockPassBuffers {
     unsigned long long *trace = nullptr;   // optional task timeline (4 words per task)
     // Batches of independent systems (nullptr for a single system).
     const int *blk_sys = nullptr, *sys_k = nullptr;
-    const long long *sys_vbase = nullptr, *sys_pbase = nullptr;
+    const long long *sys_vbase = nullptr, *sys_pbase = nullptr, *sys_sbase = nullptr;
 };
 // Dynamic LDS of a block pass: one chunk of the vector it stages (r_B of
 // up to need_rows = max m forward, [y_B; z_R] of up to max m + |R| backward),

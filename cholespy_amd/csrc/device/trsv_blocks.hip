@@ -35,7 +35,10 @@ namespace {
 
 constexpr int kLanes = 64;      // forward: most rows of M_B per task
 constexpr int kBLanes = 32;     // backward: most columns of M_B per task
-constexpr int kMU = 16;         // M loads issued together per thread
+#ifndef CHOLESPY_KMU
+#define CHOLESPY_KMU 16
+#endif
+constexpr int kMU = CHOLESPY_KMU;   // M loads issued together per thread
 
 template <typename F, int KT>
 __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a) {
@@ -51,8 +54,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
-        const int kind = a.task[4 * tk + 3];
+        const Task task = load_task(a, tk);
+        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
         // Lane layout of the product: `lanes` rows x `segs` column segments.
         const int lanes = last - first <= 16 ? 16 : (last - first <= 32 ? 32 : kLanes);
         const int segs = kTaskThreads / lanes;
@@ -66,8 +69,25 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
         const F *bv = a.b + vm.vbase;
+        F *sp = a.spart + vm.sbase;
         const int rho = first + lr;
-        for (int t0 = 0; t0 < k; t0 += KT) {
+        if (kind == 4) {   // combine the split results of rows [first, last)
+            const long long sbase = a.rptr[blk];
+            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit, [&](int i, int t, F v) {
+                const int r = first + i;
+                if (r < m) {
+                    const long long o = (long long)(a0 + r) * k + t;
+                    yv[o] = v;
+                    zv[o] = from_bits<F>(Bits<F>::sentinel);
+                } else {
+                    publish(&pv[(sbase + r - m) * k + t], v);
+                }
+            });
+            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+        }
+        // Reduction range: all columns, or [lo, hi) for a split task.
+        const int clo = kind == 3 ? task.lo : 0, chi = kind == 3 ? task.hi : m;
+        for (int t0 = 0; t0 < (kind == 4 ? 0 : k); t0 += KT) {
             const int kt = min(KT, k - t0);
             if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
                 gather_partials(a, bv, pv, rb, (F *)nullptr, a0, first, last, t0, kt, k, KT, KT, true);
@@ -77,8 +97,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
             F acc[KT];
 #pragma unroll
             for (int t = 0; t < KT; ++t) acc[t] = F(0);
-            for (int c0 = 0; c0 < m; c0 += a.chunk) {
-                const int c1 = min(m, c0 + a.chunk);
+            for (int c0 = clo; c0 < chi; c0 += a.chunk) {
+                const int c1 = min(chi, c0 + a.chunk);
                 if (kind == 0) {
                     gather_partials(a, bv, pv, rb, rv, a0, c0, c1, t0, kt, k, KT, KT, false);
                 } else {   // r_B from the gather tasks of this block (published, waited on)
@@ -88,13 +108,14 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     });
                 }
                 __syncthreads();
-                if (a.trace && tid == 0 && t0 == 0 && c0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+                if (a.trace && tid == 0 && t0 == 0 && c0 == clo) a.trace[4 * (size_t)tk + 1] = wall_clock64();
                 if (rho < last) {
-                    // Columns [c0, c1) split evenly over the segments (all lanes of
-                    // a wavefront load the same column: one coalesced access);
-                    // row rho uses columns c < cend.
+                    // The columns the task's rows use, [c0, min(c1, cuse)), split
+                    // evenly over the segments (all lanes of a wavefront load the
+                    // same column: one coalesced access); row rho uses c < cend.
+                    const int cuse = min(c1, min(last, m));
                     const int cend = min(rho < m ? rho + 1 : m, c1);
-                    const int cq = (c1 - c0 + segs - 1) / segs;
+                    const int cq = max(0, (cuse - c0 + segs - 1) / segs);
                     const int cb = c0 + seg * cq, ce = min(cend, cb + cq);
                     // Column c of the packed column-major M_B starts (rows - c)
                     // entries after column c - 1: walk the offsets incrementally,
@@ -129,7 +150,9 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     F s = rv[lr * KT + t];
                     for (int g = 1; g < segs; ++g) s += rv[(g * lanes + lr) * KT + t];
                     const int col = t0 + t;
-                    if (rho < m) {
+                    if (kind == 3) {
+                        publish(&sp[(long long)(task.sbase + rho - first) * k + col], s);
+                    } else if (rho < m) {
                         const long long o = (long long)(a0 + rho) * k + col;
                         yv[o] = s;
                         zv[o] = from_bits<F>(Bits<F>::sentinel);
@@ -166,7 +189,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
+        const Task task = load_task(a, tk);
+        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
         // Lane layout of the product: `lanes` columns x `segs` row segments.
         const int lanes = last - first <= 16 ? 16 : kBLanes;
         const int segs = kTaskThreads / lanes;
@@ -180,14 +204,33 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         const VecMap vm = vec_map(a, blk);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+        F *sp = a.spart + vm.sbase;
         const int c = first + lr;
-        for (int t0 = 0; t0 < k; t0 += KT) {
+        // Finishes column cc, right-hand side col: z published, x written.
+        auto finish = [&](int cc, int col, F s) {
+            publish(&zv[(long long)(a0 + cc) * k + col], s);
+            const long long o = vm.vbase + (long long)a.perm[a0 + cc] * k + col;
+            if (a.xmode == 0) {
+                static_cast<F *>(a.x)[o] = s;
+            } else {
+                double *x64 = static_cast<double *>(a.x);
+                x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
+            }
+        };
+        if (kind == 2) {   // combine the split results of columns [first, last)
+            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit,
+                           [&](int i, int t, F v) { finish(first + i, t, v); });
+            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+        }
+        // Reduction range: all rows, or [lo, hi) for a split task.
+        const int hlo = kind == 1 ? task.lo : 0, hhi = kind == 1 ? task.hi : rows;
+        for (int t0 = 0; t0 < (kind == 2 ? 0 : k); t0 += KT) {
             const int kt = min(KT, k - t0);
             F acc[KT];
 #pragma unroll
             for (int t = 0; t < KT; ++t) acc[t] = F(0);
-            for (int h0 = 0; h0 < rows; h0 += a.chunk) {
-                const int h1 = min(rows, h0 + a.chunk);
+            for (int h0 = hlo; h0 < hhi; h0 += a.chunk) {
+                const int h1 = min(hhi, h0 + a.chunk);
                 // v = [y_B ; z_R]: y from the forward pass (complete), z waited on.
                 gather_rows(a, vs, h0, h1, kt, KT, KT, [&](int rho, int t, bool &pub) {
                     pub = rho >= m;
@@ -195,14 +238,17 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                                : yv + (long long)(a0 + rho) * k + t0 + t;
                 });
                 __syncthreads();
-                if (a.trace && tid == 0 && t0 == 0 && h0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+                if (a.trace && tid == 0 && t0 == 0 && h0 == hlo) a.trace[4 * (size_t)tk + 1] = wall_clock64();
                 if (c < last) {
                     // Rows [h0, h1) split evenly over the segments; column c
                     // uses rows rho >= c. Row rho of the packed row-major M_B
                     // is min(rho + 1, m) entries long: walk the offsets
                     // incrementally, kMU loads in flight per batch.
-                    const int rq = (h1 - h0 + segs - 1) / segs;
-                    const int rb = max(h0 + seg * rq, c), re = min(h1, h0 + seg * rq + rq);
+                    // Rows [max(h0, first), h1) split evenly over the segments
+                    // (rows below the task's first column are zero in its columns).
+                    const int hb = max(h0, first);
+                    const int rq = max(0, (h1 - hb + segs - 1) / segs);
+                    const int rb = max(hb + seg * rq, c), re = min(h1, hb + seg * rq + rq);
                     int64_t o = rb < re ? bwd_off(m, rb, c) : 0;
                     for (int rho = rb; rho < re; rho += kMU) {
                         F mv[kMU];
@@ -233,20 +279,17 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     F s = vs[lr * KT + t];
                     for (int g = 1; g < segs; ++g) s += vs[(g * lanes + lr) * KT + t];
                     const int col = t0 + t;
-                    publish(&zv[(long long)(a0 + c) * k + col], s);
-                    const long long o = vm.vbase + (long long)a.perm[a0 + c] * k + col;
-                    if (a.xmode == 0) {
-                        static_cast<F *>(a.x)[o] = s;
-                    } else {
-                        double *x64 = static_cast<double *>(a.x);
-                        x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
-                    }
+                    if (kind == 1)
+                        publish(&sp[(long long)(task.sbase + c - first) * k + col], s);
+                    else
+                        finish(c, col, s);
                 }
             }
             __syncthreads();
         }
-        // Restore the sentinel in this block's partial slots for the next solve.
-        if (first == 0) {
+        // Restore the sentinel in this block's partial slots for the next solve
+        // (once per block: its first column task, or that task's combine).
+        if (first == 0 && kind != 1) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
             for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
             F *rb = a.rbuf + vm.vbase;
@@ -296,6 +339,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.z = static_cast<F *>(bf.z);
     a.part = static_cast<F *>(bf.part);
     a.rbuf = static_cast<F *>(bf.rbuf);
+    a.spart = static_cast<F *>(bf.spart);
     a.x = bf.x;
     a.status = bf.status;
     a.xmode = bf.xmode;
@@ -304,6 +348,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.sys_k = bf.sys_k;
     a.sys_vbase = bf.sys_vbase;
     a.sys_pbase = bf.sys_pbase;
+    a.sys_sbase = bf.sys_sbase;
     queues.fill(a.qa);
     if (tile == kPanelTile) {
         if constexpr (sizeof(F) == 4) return launch_panel_f32(forward, a, grid, stream);

@@ -109,6 +109,89 @@ __device__ __forceinline__ int panel_chunk(int lds_bytes, int ks) {
     return (lds_bytes / (ks * (int)sizeof(F)) - 4) & ~3;
 }
 
+
+// Staged gather of r_B rows [r0, r1) over all k columns (wide right-hand
+// sides): per round of up to kStageRows rows, one thread per row loads its
+// perm entry and up to kStageSlots contribution slots into LDS; then every
+// thread issues the loads of kStageElems (row, column) elements at once
+// (the b entry and the published partials, coalesced along the columns),
+// so a round costs three dependent memory latencies however wide k is.
+// to_rbuf: publish the sums to rb; otherwise store them to dst[i * stride + t].
+constexpr int kStageRows = 128;
+constexpr int kStageSlots = 4;
+// Elements per thread per load round: 4 (f32) or 2 (f64) entries, each with
+// its kStageSlots partials, bounds the registers this phase adds.
+template <typename F> constexpr int stage_elems() { return sizeof(F) == 4 ? 4 : 2; }
+
+struct GatherStage {
+    int perm[kStageRows];
+    int cnt[kStageRows];
+    int first[kStageRows];
+    int slot[kStageRows][kStageSlots];
+};
+
+template <typename F>
+__device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherStage &st, const F *bv, const F *pv,
+                                            F *rb, F *dst, int a0, int r0, int r1, int k, int stride,
+                                            bool to_rbuf) {
+    using U = typename Bits<F>::U;
+    constexpr int kStageElems = stage_elems<F>();
+    const int tid = threadIdx.x;
+    for (int q0 = r0; q0 < r1; q0 += kStageRows) {
+        const int nr = min(kStageRows, r1 - q0);
+        if (tid < nr) {
+            const int row = a0 + q0 + tid;
+            const int p0 = a.cptr[row], p1 = a.cptr[row + 1];
+            st.perm[tid] = a.perm[row];
+            st.cnt[tid] = p1 - p0;
+            st.first[tid] = p0;
+#pragma unroll
+            for (int j = 0; j < kStageSlots; ++j) st.slot[tid][j] = p0 + j < p1 ? a.cslot[p0 + j] : -1;
+        }
+        __syncthreads();
+        const int total = nr * k;
+        for (int base = tid; base < total; base += kTaskThreads * kStageElems) {
+            U ub[kStageElems], us[kStageElems][kStageSlots];
+            int ii[kStageElems], tt[kStageElems];
+#pragma unroll
+            for (int e = 0; e < kStageElems; ++e) {
+                const int idx = base + e * kTaskThreads;
+                const bool ok = idx < total;
+                ii[e] = ok ? idx / k : 0;
+                tt[e] = ok ? idx - ii[e] * k : 0;
+                ub[e] = ok ? as_bits(bv[(long long)st.perm[ii[e]] * k + tt[e]]) : U(0);
+#pragma unroll
+                for (int j = 0; j < kStageSlots; ++j) {
+                    const int sl = ok ? st.slot[ii[e]][j] : -1;
+                    us[e][j] = sl >= 0 ? load_bits(&pv[(long long)sl * k + tt[e]]) : U(0);
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < kStageElems; ++e) {
+                const int idx = base + e * kTaskThreads;
+                if (idx >= total) break;
+                const int i = ii[e], t = tt[e];
+                F v = from_bits<F>(ub[e]);
+#pragma unroll
+                for (int j = 0; j < kStageSlots; ++j) {
+                    const int sl = st.slot[i][j];
+                    if (sl >= 0) v += ready_value(&pv[(long long)sl * k + t], us[e][j], a.status);
+                }
+                for (int j = kStageSlots; j < st.cnt[i]; ++j) {   // rows with many contributions
+                    const F *p = &pv[(long long)a.cslot[st.first[i] + j] * k + t];
+                    v += ready_value(p, load_bits(p), a.status);
+                }
+                if (to_rbuf)
+                    publish(&rb[(long long)(a0 + q0 + i) * k + t], v);
+                else
+                    dst[(q0 - r0 + i) * stride + t] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+
 // Zero rows [len, len + 4) of a staged chunk.
 template <typename F>
 __device__ __forceinline__ void zero_tail(F *v, int len, int ks) {
@@ -135,6 +218,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
     extern __shared__ __align__(16) unsigned char smem[];
     F *rv = reinterpret_cast<F *>(smem);
     __shared__ int s_next;
+    __shared__ GatherStage stage;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -142,8 +226,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
-        const int kind = a.task[4 * tk + 3];
+        const Task task = load_task(a, tk);
+        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
         const int a0 = a.start[blk];
         const int m = a.start[blk + 1] - a0;
         const int rows = m + (a.rptr[blk + 1] - a.rptr[blk]);
@@ -151,9 +235,23 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
         const VecMap vm = vec_map(a, blk);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
+        F *sp = a.spart + vm.sbase;
         const F *bv = a.b + vm.vbase;
-        if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
-            gather_partials(a, bv, pv, rb, (F *)nullptr, a0, first, last, 0, k, k, k, k, true);
+        if (kind == 4) {   // combine the split results of rows [first, last)
+            const long long sbase = a.rptr[blk];
+            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit, [&](int i, int t, F v) {
+                const int r = first + i;
+                if (r < m) {
+                    const long long o = (long long)(a0 + r) * k + t;
+                    yv[o] = v;
+                    zv[o] = from_bits<F>(Bits<F>::sentinel);
+                } else {
+                    publish(&pv[(sbase + r - m) * k + t], v);
+                }
+            });
+            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+        } else if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
+            gather_wide(a, stage, bv, pv, rb, (F *)nullptr, a0, first, last, k, k, true);
             if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
         } else if (k > 0) {
             const int nT = (k + 15) / 16;
@@ -170,44 +268,80 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
             Acc acc[kMaxTiles];
 #pragma unroll
             for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
-            for (int c0 = 0; c0 < m; c0 += chunk) {
-                const int c1 = min(m, c0 + chunk);
-                if (kind == 0) {
-                    gather_partials(a, bv, pv, rb, rv, a0, c0, c1, 0, k, k, k, ks, false);
-                } else {   // r_B from the gather tasks of this block (published, waited on)
-                    gather_rows(a, rv, c0, c1, k, k, ks, [&](int i, int t, bool &pub) {
-                        pub = true;
-                        return rb + (long long)(a0 + i) * k + t;
-                    });
-                }
-                zero_tail(rv, c1 - c0, ks);
-                __syncthreads();
-                if (a.trace && tid == 0 && c0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-                if (w.ntiles > 0) {
-                    const int lim = min(rowlim, c1);
-                    const int nsteps = (min(c1, cmax) - c0 + 3) / 4;
-                    const F *bl = rv + kk * ks + (lane & 15) + 16 * w.tile0;
-                    for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
-                        F av[kSteps];
+            if (kind == 0) {
+                // Small block: r_B summed from the partials into LDS, in chunks.
+                for (int c0 = 0; c0 < m; c0 += chunk) {
+                    const int c1 = min(m, c0 + chunk);
+                    gather_wide(a, stage, bv, pv, rb, rv, a0, c0, c1, k, ks, false);
+                    zero_tail(rv, c1 - c0, ks);
+                    __syncthreads();
+                    if (a.trace && tid == 0 && c0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+                    if (w.ntiles > 0) {
+                        const int lim = min(rowlim, c1);
+                        const int nsteps = (min(c1, cmax) - c0 + 3) / 4;
+                        const F *bl = rv + kk * ks + (lane & 15) + 16 * w.tile0;
+                        for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
+                            F av[kSteps];
 #pragma unroll
-                        for (int q = 0; q < kSteps; ++q) {
-                            const int s = s0 + q * w.nsplit;
-                            const int col = c0 + 4 * s + kk;
-                            av[q] = (s < nsteps && row < last && col < lim) ? M[fwd_off32(rows, col, row)] : F(0);
-                        }
+                            for (int q = 0; q < kSteps; ++q) {
+                                const int s = s0 + q * w.nsplit;
+                                const int col = c0 + 4 * s + kk;
+                                av[q] = (s < nsteps && row < last && col < lim) ? M[fwd_off32(rows, col, row)] : F(0);
+                            }
 #pragma unroll
-                        for (int q = 0; q < kSteps; ++q) {
-                            const int s = s0 + q * w.nsplit;
-                            if (s < nsteps) {
-                                const F *bp = bl + 4 * s * ks;
+                            for (int q = 0; q < kSteps; ++q) {
+                                const int s = s0 + q * w.nsplit;
+                                if (s < nsteps) {
+                                    const F *bp = bl + 4 * s * ks;
 #pragma unroll
-                                for (int j = 0; j < kMaxTiles; ++j)
-                                    if (j < w.ntiles) acc[j] = Mfma<F>::step(av[q], bp[16 * j * w.tstride], acc[j]);
+                                    for (int j = 0; j < kMaxTiles; ++j)
+                                        if (j < w.ntiles) acc[j] = Mfma<F>::step(av[q], bp[16 * j * w.tstride], acc[j]);
+                                }
                             }
                         }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
+            } else if (w.ntiles > 0) {
+                // Large block: B = r_B read straight from rbuf (published by the
+                // block's gather tasks), no staging and no barriers; columns
+                // [lo, hi) only for a split task.
+                using U = typename Bits<F>::U;
+                const int clo = kind == 3 ? task.lo : 0;
+                const int nsteps = (min(kind == 3 ? task.hi : m, cmax) - clo + 3) / 4;
+                const F *rb0 = rb + (long long)(a0 + clo) * k + (lane & 15) + 16 * w.tile0;
+                for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
+                    F av[kSteps];
+                    U bu[kSteps][kMaxTiles];
+#pragma unroll
+                    for (int q = 0; q < kSteps; ++q) {
+                        const int s = s0 + q * w.nsplit;
+                        const int col = clo + 4 * s + kk;
+                        const int chi = kind == 3 ? task.hi : m;
+                        av[q] = (s < nsteps && row < last && col < rowlim && col < chi) ? M[fwd_off32(rows, col, row)]
+                                                                                       : F(0);
+                        const bool bok = s < nsteps && col < chi;
+#pragma unroll
+                        for (int j = 0; j < kMaxTiles; ++j) {
+                            const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
+                            bu[q][j] = (j < w.ntiles && bok && t < k)
+                                           ? load_bits(rb0 + (long long)(col - clo) * k + 16 * j * w.tstride) : U(0);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < kSteps; ++q) {
+                        const int s = s0 + q * w.nsplit;
+                        if (s < nsteps) {
+                            const int col = 4 * s + kk;   // relative to clo
+#pragma unroll
+                            for (int j = 0; j < kMaxTiles; ++j)
+                                if (j < w.ntiles)
+                                    acc[j] = Mfma<F>::step(
+                                        av[q], ready_value(rb0 + (long long)col * k + 16 * j * w.tstride, bu[q][j], a.status),
+                                        acc[j]);
+                        }
+                    }
+                }
             }
             if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
             if (w.ntiles > 0 && w.split == 0) {
@@ -222,7 +356,9 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
                         const int rho = g0 + Mfma<F>::row(kk, v);
                         if (rho >= last) break;
                         const F s = acc[j][v];
-                        if (rho < m) {
+                        if (kind == 3) {
+                            publish(&sp[(long long)(task.sbase + rho - first) * k + t], s);
+                        } else if (rho < m) {
                             const long long o = (long long)(a0 + rho) * k + t;
                             yv[o] = s;
                             zv[o] = from_bits<F>(Bits<F>::sentinel);
@@ -258,7 +394,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
         int pre = 0;
         if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-        const int blk = a.task[4 * tk], first = a.task[4 * tk + 1], last = a.task[4 * tk + 2];
+        const Task task = load_task(a, tk);
+        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
         const int a0 = a.start[blk];
         const int m = a.start[blk + 1] - a0;
         const int r0 = a.rptr[blk];
@@ -268,10 +405,24 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
         const VecMap vm = vec_map(a, blk);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
-        if (k > 0) {
+        F *sp = a.spart + vm.sbase;
+        // Finishes column cc, right-hand side t: z published, x written.
+        auto finish = [&](int cc, int t, F s) {
+            publish(&zv[(long long)(a0 + cc) * k + t], s);
+            const long long o = vm.vbase + (long long)a.perm[a0 + cc] * k + t;
+            if (a.xmode == 0) {
+                static_cast<F *>(a.x)[o] = s;
+            } else {
+                double *x64 = static_cast<double *>(a.x);
+                x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
+            }
+        };
+        if (kind == 2) {   // combine the split results of columns [first, last)
+            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit,
+                           [&](int i, int t, F v) { finish(first + i, t, v); });
+            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+        } else if (k > 0) {
             const int nT = (k + 15) / 16;
-            const int ks = panel_stride(k);
-            const int chunk = panel_chunk<F>(a.lds_bytes, ks);
             const WaveWork w = wave_work(wave, last - first, nT);
             const int cg0 = first + 16 * max(w.group, 0);
             const int c = cg0 + (lane & 15);   // A row (= column of M_B) of this lane
@@ -279,42 +430,57 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
             Acc acc[kMaxTiles];
 #pragma unroll
             for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
-            for (int h0 = 0; h0 < rows; h0 += chunk) {
-                const int h1 = min(rows, h0 + chunk);
-                // v = [y_B ; z_R]: y from the forward pass (complete), z waited on.
-                gather_rows(a, vs, h0, h1, k, k, ks, [&](int rho, int t, bool &pub) {
-                    pub = rho >= m;
-                    return pub ? zv + (long long)a.rlist[r0 + rho - m] * k + t : yv + (long long)(a0 + rho) * k + t;
-                });
-                zero_tail(vs, h1 - h0, ks);
-                __syncthreads();
-                if (a.trace && tid == 0 && h0 == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-                if (w.ntiles > 0) {
-                    // Rows below the group's first column are zero in its columns.
-                    const int hs = max(h0, cg0);
-                    const int nsteps = (h1 - hs + 3) / 4;
-                    const F *bl = vs + (hs - h0 + kk) * ks + (lane & 15) + 16 * w.tile0;
-                    for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
-                        F av[kSteps];
+            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+            if (w.ntiles > 0) {
+                // B = v[rho] = y_B[rho] (rho < m, complete since the forward
+                // pass) or z of R row rlist[rho - m] (published, waited on),
+                // read straight from global memory. Rows below the group's
+                // first column are zero in its columns.
+                using U = typename Bits<F>::U;
+                const int hs = kind == 1 ? max(cg0, task.lo) : cg0;   // rows [hs, he)
+                const int he = kind == 1 ? task.hi : rows;
+                const int nsteps = (he - hs + 3) / 4;
+                const int toff = (lane & 15) + 16 * w.tile0;
+                for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
+                    F av[kSteps];
+                    const F *bp[kSteps];
+                    bool pubr[kSteps];
 #pragma unroll
-                        for (int q = 0; q < kSteps; ++q) {
-                            const int s = s0 + q * w.nsplit;
-                            const int rho = hs + 4 * s + kk;
-                            av[q] = (s < nsteps && c < last && rho < h1 && rho >= c) ? M[bwd_off32(m, rho, c)] : F(0);
+                    for (int q = 0; q < kSteps; ++q) {
+                        const int s = s0 + q * w.nsplit;
+                        const int rho = hs + 4 * s + kk;
+                        av[q] = (s < nsteps && c < last && rho < he && rho >= c) ? M[bwd_off32(m, rho, c)] : F(0);
+                        pubr[q] = rho >= m;
+                        bp[q] = nullptr;
+                        if (s < nsteps && rho < he)
+                            bp[q] = (rho < m ? yv + (long long)(a0 + rho) * k
+                                             : zv + (long long)a.rlist[r0 + rho - m] * k) + toff;
+                    }
+                    U bu[kSteps][kMaxTiles];
+#pragma unroll
+                    for (int q = 0; q < kSteps; ++q)
+#pragma unroll
+                        for (int j = 0; j < kMaxTiles; ++j) {
+                            const int t = toff + 16 * j * w.tstride;
+                            const bool ok = j < w.ntiles && bp[q] && t < k;
+                            bu[q][j] = !ok ? U(0)
+                                           : (pubr[q] ? load_bits(bp[q] + 16 * j * w.tstride)
+                                                      : as_bits(bp[q][16 * j * w.tstride]));
                         }
 #pragma unroll
-                        for (int q = 0; q < kSteps; ++q) {
-                            const int s = s0 + q * w.nsplit;
-                            if (s < nsteps) {
-                                const F *bp = bl + 4 * s * ks;
+                    for (int q = 0; q < kSteps; ++q) {
+                        const int s = s0 + q * w.nsplit;
+                        if (s < nsteps) {
 #pragma unroll
-                                for (int j = 0; j < kMaxTiles; ++j)
-                                    if (j < w.ntiles) acc[j] = Mfma<F>::step(av[q], bp[16 * j * w.tstride], acc[j]);
-                            }
+                            for (int j = 0; j < kMaxTiles; ++j)
+                                if (j < w.ntiles) {
+                                    const F b = pubr[q] && bp[q] ? ready_value(bp[q] + 16 * j * w.tstride, bu[q][j], a.status)
+                                                                 : from_bits<F>(bu[q][j]);
+                                    acc[j] = Mfma<F>::step(av[q], b, acc[j]);
+                                }
                         }
                     }
                 }
-                __syncthreads();
             }
             if (w.nsplit > 1) reduce_splits(vs, w, wave, lane, acc[0]);
             if (w.ntiles > 0 && w.split == 0) {
@@ -328,20 +494,17 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                         const int cc = cg0 + Mfma<F>::row(kk, v);
                         if (cc >= last) break;
                         const F s = acc[j][v];
-                        publish(&zv[(long long)(a0 + cc) * k + t], s);
-                        const long long o = vm.vbase + (long long)a.perm[a0 + cc] * k + t;
-                        if (a.xmode == 0) {
-                            static_cast<F *>(a.x)[o] = s;
-                        } else {
-                            double *x64 = static_cast<double *>(a.x);
-                            x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
-                        }
+                        if (kind == 1)
+                            publish(&sp[(long long)(task.sbase + cc - first) * k + t], s);
+                        else
+                            finish(cc, t, s);
                     }
                 }
             }
         }
-        // Restore the sentinel in this block's partial slots for the next solve.
-        if (first == 0) {
+        // Restore the sentinel in this block's partial slots for the next solve
+        // (once per block: its first column task, or that task's combine).
+        if (first == 0 && kind != 1) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
             for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
             F *rb = a.rbuf + vm.vbase;

@@ -118,6 +118,21 @@ py::object as_exportable(py::handle obj) {
     return py::module_::import("numpy").attr("asarray")(obj);
 }
 
+// Plan arrays of a GPU handle (cholespy_solver_plan_arrays) as a dict of numpy arrays.
+py::dict plan_arrays(const cholespy_gpu_solver *h) {
+    const char *names[] = {"start", "rptr", "level_fwd", "level_bwd", "fwd_task", "bwd_task"};
+    py::dict d;
+    for (int w = 0; w < 6; ++w) {
+        int64_t size = 0;
+        throw_on_error(cholespy_solver_plan_arrays(h, w, nullptr, 0, &size));
+        py::array_t<int32_t> a(size);
+        throw_on_error(cholespy_solver_plan_arrays(h, w, a.mutable_data(), size, nullptr));
+        if (w >= 4) a.resize(std::vector<py::ssize_t>{size / CHOLESPY_TASK_INTS, CHOLESPY_TASK_INTS});
+        d[names[w]] = a;
+    }
+    return d;
+}
+
 template <typename T>
 T read_elem(const char *p, const dl::DataType &dt) {
     switch (dt.code) {
@@ -299,6 +314,7 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_check", [](const Class &self) { self.check(); })
         .def("_set_trace", [](Class &self, bool on) { throw_on_error(cholespy_solver_set_trace(self.gpu(), on ? 1 : 0)); },
              py::arg("on"))
+        .def("_plan_arrays", [](const Class &self) { return plan_arrays(self.gpu()); })
         .def("_trace",
              [](Class &self, bool forward) {
                  const std::vector<int64_t> info = self.plan_info();
@@ -421,6 +437,19 @@ void declare_batch(py::module_ &m, const char *name) {
                  for (size_t i = 0; i < v.size(); ++i) d[names[i]] = v[i];
                  return d;
              })
+        .def("_plan_arrays", [](const Class &self) { return plan_arrays(self.gpu()); })
+        .def("_set_trace", [](Class &self, bool on) { throw_on_error(cholespy_solver_set_trace(self.gpu(), on ? 1 : 0)); },
+             py::arg("on"))
+        .def("_trace",
+             [](Class &self, bool forward) {
+                 std::vector<int64_t> info(8, 0);
+                 throw_on_error(cholespy_solver_plan_info(self.gpu(), info.data(), (int)info.size()));
+                 const int64_t words = 4 * (forward ? info[4] : info[5]);
+                 py::array_t<uint64_t> out(words);
+                 throw_on_error(cholespy_solver_get_trace(self.gpu(), forward ? 1 : 0, out.mutable_data(), words));
+                 return out;
+             },
+             py::arg("forward"))
         .def_property_readonly("sizes", [](const Class &self) { return self.sizes(); });
 }
 

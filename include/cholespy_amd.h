@@ -171,6 +171,16 @@ int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);
  * tasks, partial slots, largest m + |R|}. No counterpart in the reference. */
 int cholespy_solver_plan_info(const cholespy_gpu_solver *solver, int64_t *info, int count);
 
+/* Copies of the block plan (diagnostics; no counterpart in the reference):
+ * which = 0 block row starts (blocks + 1), 1 partial-slot ranges
+ * (blocks + 1), 2 forward block levels (blocks), 3 backward block levels
+ * (blocks), 4 forward tasks and 5 backward tasks (CHOLESPY_TASK_INTS ints
+ * each: block, first, last, kind, lo, hi, split-row base, splits). Copies
+ * min(count, size) ints to out; *size (if not null) receives the full size. */
+#define CHOLESPY_TASK_INTS 8
+int cholespy_solver_plan_arrays(const cholespy_gpu_solver *solver, int which, int32_t *out, int64_t count,
+                                int64_t *size);
+
 /* Gradient of a loss with respect to the caller's stored matrix entries
  * (rows[e], cols[e], e < nnz, device int32 arrays) for x = A^{-1} b, given
  * x and the adjoint lam = A^{-1} dL/dx (device, row-major n x n_rhs,

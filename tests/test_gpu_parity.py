@@ -175,3 +175,29 @@ def test_single_passes_compose_to_a_solve():
         torch.cuda.synchronize()
         solver._check()
         assert torch.equal(x1, x2)
+
+
+@pytest.mark.parametrize("split_entries", [512, 2048])
+@pytest.mark.parametrize("case", ["ico5", "grid64"])
+@pytest.mark.parametrize("cls,dtype", [(CholeskySolverF, np.float32), (CholeskySolverD, np.float64)])
+def test_split_reductions_match_oracle(case, cls, dtype, split_entries, monkeypatch):
+    """Plans whose large product tasks are split along their reduction (split
+    tasks + combine tasks in both passes) give the same solutions; both the
+    vector-FMA (k <= 4) and the MFMA (k > 4) kernels run them."""
+    monkeypatch.setenv("CHOLESPY_SPLIT_ENTRIES", str(split_entries))
+    n, values, idx = CASES[case]()
+    solver = _gpu_solver(cls, n, values, idx)
+    fac = solver._factor()
+    tasks = solver._plan_arrays()
+    assert (tasks["fwd_task"][:, 3] == 3).any() and (tasks["bwd_task"][:, 3] == 1).any()
+    rng = np.random.default_rng(7)
+    for k in (3, 17, 64, 3):
+        b = rng.random((n, k)).astype(dtype)
+        bt = torch.tensor(b, device=DEV)
+        xt = torch.zeros_like(bt)
+        solver.solve(bt, xt)
+        torch.cuda.synchronize()
+        solver._check()
+        ref = oracle.solve(fac, b, dtype)
+        err = np.abs(xt.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < TOL[dtype], (case, k, err)
