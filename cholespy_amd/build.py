@@ -36,7 +36,7 @@ HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare", "-pthr
 DEVICE_SOURCES = ["analysis.hip", "trsv_rows.hip", "trsv_blocks.hip", "trsv_panel.hip", "refine.hip", "grad.hip",
                   "capi.hip"]
 PLAN_SOURCES = ["block_plan.cpp"]   # host code linked into libcholespy_amd.so
-HOST_SOURCES = ["sparse_input.cpp", "ordering.cpp", "factor.cpp", "cholesky_solver.cpp",
+HOST_SOURCES = ["sparse_input.cpp", "ordering.cpp", "factor.cpp", "supernodal.cpp", "cholesky_solver.cpp",
                 "bindings.cpp"]
 
 LIB_NAME = "libcholespy_amd.so"

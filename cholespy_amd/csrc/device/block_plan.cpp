@@ -1,6 +1,8 @@
 #include "block_plan.hpp"
 
+#include <atomic>
 #include <functional>
+#include <thread>
 
 #include <algorithm>
 #include <numeric>
@@ -62,7 +64,7 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     // 2. Per block: R rows, dense L_BB and L_RB, inverse and products.
     P.rptr.assign((size_t)nb + 1, 0);
     P.moff.assign((size_t)nb + 1, 0);
-    std::vector<int> mark(n, -1), pos(n, -1);
+    std::vector<int> mark(n, -1);
     std::vector<int> rows_b;
     for (int b = 0; b < nb; ++b) {
         const int a = start[b], e = start[b + 1];
@@ -85,9 +87,19 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     }
     P.mfwd.assign((size_t)P.moff[nb], 0.0);
     P.mbwd.assign((size_t)P.moff[nb], 0.0);
-    {
+    // Dense inverses per block: independent, so blocks are shared out over
+    // host threads (heaviest first, taken from a shared counter).
+    std::vector<int> by_cost(nb);
+    std::iota(by_cost.begin(), by_cost.end(), 0);
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](int x, int y) {
+        return P.moff[x + 1] - P.moff[x] > P.moff[y + 1] - P.moff[y];
+    });
+    std::atomic<int> next_block(0);
+    auto dense_work = [&]() {
+        std::vector<int> pos(n, -1);
         std::vector<double> Lbb, Pb, Linv, Q;
-        for (int b = 0; b < nb; ++b) {
+        for (int t = next_block.fetch_add(1); t < nb; t = next_block.fetch_add(1)) {
+            const int b = by_cost[t];
             const int a = start[b], e = start[b + 1];
             const int m = e - a;
             const int R = P.rptr[b + 1] - P.rptr[b];
@@ -137,6 +149,14 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
             }
             for (int q = 0; q < R; ++q) pos[rl[q]] = -1;
         }
+    };
+    {
+        const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const int nt = std::max(1, std::min(hw, opt.threads > 0 ? opt.threads : hw));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(dense_work);
+        dense_work();
+        for (auto &t : pool) t.join();
     }
 
     // 3. Contribution lists: slot s (row rlist[s]) feeds row rlist[s].

@@ -38,6 +38,9 @@ struct BlockPlanOptions {
     // whose partial results a combine task adds (0 = no splitting).
     int split_entries = 4096;
     int max_split = 16;
+    // Host threads for the dense block inverses (0 = hardware count, at most
+    // 16). Batches already build one plan per thread and pass 1.
+    int threads = 0;
 };
 
 struct BlockPlan {

@@ -827,7 +827,8 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         if ((int)s->pending[i].colptr.size() != s->sys_n[i] + 1)
             return fail(CHOLESPY_ERR_STATE, "a system of the batch has no factor");
     // Per-system plans, built on host threads.
-    const BlockPlanOptions opt = plan_options();
+    BlockPlanOptions opt = plan_options();
+    opt.threads = 1;
     std::vector<BlockPlan> plans((size_t)ns);
     {
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));

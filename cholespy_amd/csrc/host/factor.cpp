@@ -1,10 +1,13 @@
 #include "factor.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <stdexcept>
 
 #include "ordering.hpp"
+#include "supernodal.hpp"
 
 namespace cholespy_amd {
 
@@ -188,6 +191,36 @@ Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
     f.colptr[n] = (int)total;
     f.rowidx.resize((size_t)total);
     f.val.resize((size_t)total);
+
+    FactorOptions::Method method = opt.method;
+    if (method == FactorOptions::Method::Auto) {
+        const char *e = std::getenv("CHOLESPY_FACTOR");
+        if (e && std::strcmp(e, "simplicial") == 0)
+            method = FactorOptions::Method::Simplicial;
+        else if (e && std::strcmp(e, "supernodal") == 0)
+            method = FactorOptions::Method::Supernodal;
+        else
+            method = total >= opt.supernodal_min_nnz ? FactorOptions::Method::Supernodal
+                                                     : FactorOptions::Method::Simplicial;
+    }
+    if (method == FactorOptions::Method::Supernodal) {
+        // 4'. Multifrontal numeric factorisation on the lower CSC of P A P'
+        //     (the transpose of C).
+        std::vector<int> lp((size_t)n + 1, 0), li((size_t)C.colptr[n]);
+        std::vector<double> lv((size_t)C.colptr[n]);
+        for (int k = 0; k < n; ++k)
+            for (int p = C.colptr[k]; p < C.colptr[k + 1]; ++p) lp[(size_t)C.rowidx[p] + 1]++;
+        for (int j = 0; j < n; ++j) lp[j + 1] += lp[j];
+        std::vector<int> next(lp.begin(), lp.end() - 1);
+        for (int k = 0; k < n; ++k)
+            for (int p = C.colptr[k]; p < C.colptr[k + 1]; ++p) {
+                const int q = next[C.rowidx[p]]++;
+                li[q] = k;
+                lv[q] = C.val[p];
+            }
+        supernodal_numeric(n, lp, li, lv, f.parent, count, f, opt.threads);
+        return f;
+    }
 
     // 4. Up-looking numeric factorisation: row k of L solves a sparse
     //    triangular system against the already computed columns.

@@ -30,6 +30,14 @@ struct FactorOptions {
     bool natural = false;
     int nd_leaf_size = 48;
     double nd_min_balance = 0.25;
+    // Numeric phase: simplicial up-looking (exact row-by-row), supernodal
+    // multifrontal (supernodal.hpp), or auto = supernodal once L has at
+    // least supernodal_min_nnz entries. CHOLESPY_FACTOR=simplicial|supernodal
+    // overrides auto. Both give the same pattern of L and values equal to
+    // rounding.
+    enum class Method { Auto, Simplicial, Supernodal } method = Method::Auto;
+    int64_t supernodal_min_nnz = 200000;
+    int threads = 0;   // supernodal worker threads (0 = hardware count, at most 16)
 };
 
 // Throws std::invalid_argument("Matrix is not positive definite!") when a

@@ -84,6 +84,46 @@ def test_factor_matches_dense_cholesky():
     assert np.allclose(L, Ld, rtol=1e-13, atol=1e-14)
 
 
+@pytest.mark.parametrize("method", ["simplicial", "supernodal"])
+@pytest.mark.parametrize("case", ["ico4", "grid", "cube"])
+def test_numeric_methods_agree(case, method, monkeypatch):
+    """The supernodal multifrontal numeric phase and the simplicial up-looking
+    one give the same pattern of L (fundamental supernodes add no entries)
+    and values equal to rounding, and both reproduce P A P' = L L'."""
+    if case == "ico4":
+        n, v, idx = mesh(4)
+    elif case == "grid":
+        n = 48 * 40
+        v, idx = workloads.grid2d_laplacian(48, 40, 1e-2)
+    else:
+        n = 14 ** 3
+        v, idx = workloads.grid3d_laplacian(14, 1e-3)
+    monkeypatch.setenv("CHOLESPY_FACTOR", "simplicial")
+    ref = CholeskySolverD(n, idx[0], idx[1], v, MatrixType.COO)._factor()
+    monkeypatch.setenv("CHOLESPY_FACTOR", method)
+    f = CholeskySolverD(n, idx[0], idx[1], v, MatrixType.COO)._factor()
+    for key in ("perm", "colptr", "rowidx", "parent"):
+        assert np.array_equal(f[key], ref[key]), key
+    assert np.abs(f["val"] - ref["val"]).max() <= 1e-12 * np.abs(ref["val"]).max()
+    L = sp.csc_matrix((f["val"], f["rowidx"], f["colptr"]), shape=(n, n))
+    A = sp.coo_matrix((v, (idx[0], idx[1])), shape=(n, n)).tocsc()
+    P = f["perm"]
+    assert abs(A[P][:, P] - L @ L.T).max() / abs(A).max() < 1e-13
+
+
+@pytest.mark.parametrize("method", ["simplicial", "supernodal"])
+def test_not_positive_definite_large(method, monkeypatch):
+    """An indefinite matrix large enough for dense fronts raises the same
+    error from either numeric phase."""
+    monkeypatch.setenv("CHOLESPY_FACTOR", method)
+    n = 40 * 40
+    v, idx = workloads.grid2d_laplacian(40, 40, 1e-2)
+    v = v.copy()
+    v[idx[0] == idx[1]] -= 2.5   # shifts the spectrum below zero
+    with pytest.raises(ValueError, match="Matrix is not positive definite!"):
+        CholeskySolverD(n, idx[0], idx[1], v, MatrixType.COO)
+
+
 def test_not_positive_definite():
     n = 3
     ii = np.array([0, 1, 2, 1])
