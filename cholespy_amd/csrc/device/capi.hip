@@ -826,9 +826,13 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     for (int i = 0; i < ns; ++i)
         if ((int)s->pending[i].colptr.size() != s->sys_n[i] + 1)
             return fail(CHOLESPY_ERR_STATE, "a system of the batch has no factor");
-    // Per-system plans, built on host threads.
+    // Per-system plans, built on host threads. Many systems already give
+    // every level plenty of independent tasks, so their reductions are not
+    // split (extra tasks would only add hand-offs) unless
+    // CHOLESPY_SPLIT_ENTRIES asks for it.
     BlockPlanOptions opt = plan_options();
     opt.threads = 1;
+    if (ns >= 8 && !std::getenv("CHOLESPY_SPLIT_ENTRIES")) opt.split_entries = 0;
     std::vector<BlockPlan> plans((size_t)ns);
     {
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
