@@ -27,6 +27,7 @@ struct BlockPassArgs {
     const F *b;
     F *y, *z, *part, *rbuf;
     F *spart;               // split results (split rows x k)
+    int *scnt;              // finished splits per split group (indexed by its first split row)
     void *x;
     int *status;
     int xmode;
@@ -66,7 +67,6 @@ namespace {
 
 constexpr int kTaskThreads = 256;
 constexpr int kGatherUnroll = 8;
-constexpr int kPairs = 4;       // partial-sum gather: (row, column) pairs walked together
 
 template <typename F>
 __device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, int blk) {
@@ -88,10 +88,27 @@ __device__ __forceinline__ F ready_value(const F *p, typename Bits<F>::U u, int 
     return u == Bits<F>::sentinel ? wait_value(p, status) : from_bits<F>(u);
 }
 
-// Combine task: the nsplit split results of w rows (split s, row i at split
-// row sbase + s * w + i) added in split order, so the sum does not depend on
-// timing; each split value is reset to the sentinel after it is read (the
-// combine is its only reader). out(i, t, value) finishes row i, column t.
+// Split epilogue: every split of a group counts itself done after its results
+// are published; the workgroup whose split is the last to do so learns it
+// here (the result is the same in all its threads) and resets the counter.
+template <typename F>
+__device__ __forceinline__ bool last_split(const BlockPassArgs<F> &a, int sbase, int nsplit, int *s_flag) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int old = __hip_atomic_fetch_add(a.scnt + sbase, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *s_flag = old == nsplit - 1;
+        if (*s_flag) store_int(a.scnt + sbase, 0);
+    }
+    __syncthreads();
+    return *s_flag != 0;
+}
+
+// Combine: the nsplit split results of w rows (split s, row i at split row
+// sbase + s * w + i) added in split order, so the sum does not depend on
+// which split finished last; each split value is reset to the sentinel after
+// it is read (the combine is its only reader). Values of the other splits
+// that are not yet visible are waited on. out(i, t, value) finishes row i,
+// column t.
 constexpr int kMaxSplit = 16;
 template <typename F, typename Out>
 __device__ __forceinline__ void combine_splits(const BlockPassArgs<F> &a, F *sp, int w, int k, int sbase, int nsplit,
@@ -134,16 +151,25 @@ __device__ __forceinline__ int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {
 
 // Right-hand side r_B for rows [r0, r1) of a block and columns [t0, t0 + kt):
 // (P b) plus the partials other blocks published for those rows. Each thread
-// owns up to kPairs (row, column) pairs and walks their slot lists together,
-// 4 slots per pair per round, so up to 4 * kPairs slot loads are in flight.
+// owns one or two (row, column) pairs. Per pair it loads the slot indices of up
+// to kSlotBatch contributions at once, then their values kValBatch at a time
+// (all loads of a batch in flight together), so a row with up to 16
+// contributions costs three dependent memory latencies (slot range, slot
+// indices, values) plus one per further value batch; the sum runs in slot
+// order, so it is deterministic.
 // to_rbuf: publish the sums to rb (read by the block's product tasks);
 // otherwise store them in LDS, row r0 + i at dst[i * stride + t] for
 // t < width (zeros for kt <= t < width).
+constexpr int kSlotBatch = 16;
+constexpr int kValBatch = 8;
 template <typename F>
 __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const F *bv, const F *pv, F *rb,
                                                 F *dst, int a0, int r0, int r1, int t0, int kt, int k,
                                                 int width, int stride, bool to_rbuf) {
     using U = typename Bits<F>::U;
+    // One pair per thread in FP64 keeps the gather's registers within the
+    // product loop's budget (occupancy), two in FP32.
+    constexpr int kPairs = sizeof(F) == 4 ? 2 : 1;
     const int total = (r1 - r0) * width;
     for (int base = threadIdx.x; base < total; base += kTaskThreads * kPairs) {
         int p0[kPairs], p1[kPairs], col[kPairs];
@@ -162,29 +188,40 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
                 p1[q] = a.cptr[row + 1];
             }
         }
-        for (int r = 0;; r += 4) {
+        for (int r = 0;; r += kSlotBatch) {
             bool any = false;
-            U u[kPairs][4];
-            int sl[kPairs][4];
+            int sl[kPairs][kSlotBatch];
 #pragma unroll
             for (int q = 0; q < kPairs; ++q)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < kSlotBatch; ++j) {
                     const int p = p0[q] + r + j;
                     sl[q][j] = p < p1[q] ? a.cslot[p] : -1;
-                    u[q][j] = sl[q][j] >= 0 ? load_bits(&pv[(long long)sl[q][j] * k + col[q]]) : U(0);
                     any |= sl[q][j] >= 0;
                 }
             if (!any) break;
 #pragma unroll
-            for (int q = 0; q < kPairs; ++q)
+            for (int h = 0; h < kSlotBatch; h += kValBatch) {
+                bool more = false;
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    F x = from_bits<F>(u[q][j]);
-                    if (sl[q][j] >= 0 && u[q][j] == Bits<F>::sentinel)
-                        x = wait_value(&pv[(long long)sl[q][j] * k + col[q]], a.status);
-                    v[q] += x;
-                }
+                for (int q = 0; q < kPairs; ++q) more |= sl[q][h] >= 0;
+                if (!more) break;
+                U u[kPairs][kValBatch];
+#pragma unroll
+                for (int q = 0; q < kPairs; ++q)
+#pragma unroll
+                    for (int j = 0; j < kValBatch; ++j)
+                        u[q][j] = sl[q][h + j] >= 0 ? load_bits(&pv[(long long)sl[q][h + j] * k + col[q]]) : U(0);
+#pragma unroll
+                for (int q = 0; q < kPairs; ++q)
+#pragma unroll
+                    for (int j = 0; j < kValBatch; ++j) {
+                        F x = from_bits<F>(u[q][j]);
+                        if (sl[q][h + j] >= 0 && u[q][j] == Bits<F>::sentinel)
+                            x = wait_value(&pv[(long long)sl[q][h + j] * k + col[q]], a.status);
+                        v[q] += x;
+                    }
+            }
         }
 #pragma unroll
         for (int q = 0; q < kPairs; ++q) {

@@ -219,11 +219,10 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     std::iota(order.begin(), order.end(), 0);
     std::stable_sort(order.begin(), order.end(),
                      [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
-    std::vector<int> combines;
     for (size_t lo = 0; lo < order.size();) {
         // One level at a time: the gather tasks of its large blocks first,
-        // then the product tasks, then the combine tasks, so every task's
-        // inputs come from tasks with smaller tickets.
+        // then the product tasks, so every task's inputs come from tasks with
+        // smaller tickets.
         size_t hi = lo;
         while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
         for (size_t q = lo; q < hi; ++q) {
@@ -231,7 +230,6 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
             if (m < opt.split_gather) continue;
             for (int r0 = 0; r0 < m; r0 += opt.fwd_rows) push(P.fwd_task, b, r0, std::min(m, r0 + opt.fwd_rows), 1);
         }
-        combines.clear();
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q], m = start[b + 1] - start[b];
             const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
@@ -256,13 +254,10 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
                 const int parts = (int)cb.size() - 1;
                 const int base = next_split;
                 for (int s2 = 0; s2 < parts; ++s2)
-                    push(P.fwd_task, b, r0, r1, 3, cb[s2], cb[s2 + 1], base + s2 * w, parts);
+                    push(P.fwd_task, b, r0, r1, 3, cb[s2], cb[s2 + 1], base, split_code(s2, parts));
                 next_split += parts * w;
-                const int rec[kTaskInts] = {b, r0, r1, 4, 0, 0, base, parts};
-                combines.insert(combines.end(), rec, rec + kTaskInts);
             }
         }
-        P.fwd_task.insert(P.fwd_task.end(), combines.begin(), combines.end());
         lo = hi;
     }
     // The backward pass reuses the split rows: each combine resets its rows
@@ -275,7 +270,6 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     for (size_t lo = 0; lo < order.size();) {
         size_t hi = lo;
         while (hi < order.size() && P.level_bwd[order[hi]] == P.level_bwd[order[lo]]) ++hi;
-        combines.clear();
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q], m = start[b + 1] - start[b];
             const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
@@ -296,13 +290,10 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
                 const int parts = (int)hb.size() - 1;
                 const int base = next_split;
                 for (int s2 = 0; s2 < parts; ++s2)
-                    push(P.bwd_task, b, c0, c1, 1, hb[s2], hb[s2 + 1], base + s2 * w, parts);
+                    push(P.bwd_task, b, c0, c1, 1, hb[s2], hb[s2 + 1], base, split_code(s2, parts));
                 next_split += parts * w;
-                const int rec[kTaskInts] = {b, c0, c1, 2, 0, 0, base, parts};
-                combines.insert(combines.end(), rec, rec + kTaskInts);
             }
         }
-        P.bwd_task.insert(P.bwd_task.end(), combines.begin(), combines.end());
         lo = hi;
     }
     P.nsplit_rows = std::max(fwd_split_rows, next_split);

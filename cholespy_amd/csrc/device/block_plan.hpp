@@ -20,6 +20,10 @@
 namespace cholespy_amd {
 
 constexpr int kTaskInts = 8;   // ints per task record (CHOLESPY_TASK_INTS of the C ABI)
+// Split index and split count of a split task, in one int.
+constexpr int split_code(int s, int nsplit) { return nsplit | (s << 8); }
+constexpr int split_count(int code) { return code & 255; }
+constexpr int split_index(int code) { return code >> 8; }
 
 struct BlockPlanOptions {
     int max_block = 1024;   // rows per block (dense inverse is m x m)
@@ -60,17 +64,18 @@ struct BlockPlan {
     // Block level in the block tree: forward level (height) and backward level (depth).
     std::vector<int> level_fwd, level_bwd;
     // Tasks in schedule order, kTaskInts ints each:
-    //   (block, first, last, kind, lo, hi, sbase, nsplit).
+    //   (block, first, last, kind, lo, hi, sbase, split).
     // Forward kinds: 0 = gather r_B and multiply rows [first, last) of M_B;
     // 1 = gather rows [first, last) of r_B only (published for kinds 2, 3);
     // 2 = multiply rows [first, last) of M_B by the published r_B;
-    // 3 = as 2 over columns [lo, hi) only, result published to the split
-    //     rows sbase + (rho - first) (an array of its own, k values a row);
-    // 4 = combine: add the nsplit split results of rows [first, last)
-    //     (split s at rows sbase + s * (last - first)) and finish as kind 2.
-    // Backward kinds: 0 = columns [first, last) of M_B; 1 = as 0 over rows
-    // [lo, hi) only, published to split rows sbase + (c - first); 2 =
-    // combine nsplit split results (as forward kind 4) and finish as kind 0.
+    // 3 = split s of nsplit (split = split_code(s, nsplit)): as 2 over
+    //     columns [lo, hi) only, result published to the split rows
+    //     sbase + s * (last - first) + (rho - first) (an array of its own,
+    //     k values a row); the split that finishes last (a counter per
+    //     group, indexed by sbase) adds all nsplit results in split order
+    //     and finishes the rows as kind 2 would.
+    // Backward kinds: 0 = columns [first, last) of M_B; 1 = split s of
+    // nsplit over rows [lo, hi), combined the same way and finished as 0.
     std::vector<int> fwd_task, bwd_task;
     int nsplit_rows = 0;                   // rows of the split-result array
     int max_m = 0, max_rows = 0;           // largest m and m + |R|

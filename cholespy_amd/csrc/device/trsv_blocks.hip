@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
     // One chunk of r_B (a.chunk rows x KT); after the products the same
     // space holds the 256 x KT per-thread sums.
     F *rv = reinterpret_cast<F *>(smem);
-    __shared__ int s_next;
+    __shared__ int s_next, s_flag;
     const int tid = threadIdx.x;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -71,23 +71,22 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const F *bv = a.b + vm.vbase;
         F *sp = a.spart + vm.sbase;
         const int rho = first + lr;
-        if (kind == 4) {   // combine the split results of rows [first, last)
-            const long long sbase = a.rptr[blk];
-            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit, [&](int i, int t, F v) {
-                const int r = first + i;
-                if (r < m) {
-                    const long long o = (long long)(a0 + r) * k + t;
-                    yv[o] = v;
-                    zv[o] = from_bits<F>(Bits<F>::sentinel);
-                } else {
-                    publish(&pv[(sbase + r - m) * k + t], v);
-                }
-            });
-            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-        }
-        // Reduction range: all columns, or [lo, hi) for a split task.
+        // Finishes row r, column t of the task: y (rows < m) or a partial slot.
+        auto finish = [&](int r, int t, F v) {
+            if (r < m) {
+                const long long o = (long long)(a0 + r) * k + t;
+                yv[o] = v;
+                zv[o] = from_bits<F>(Bits<F>::sentinel);
+            } else {
+                publish(&pv[(long long)(a.rptr[blk] + r - m) * k + t], v);
+            }
+        };
+        // Reduction range: all columns, or [lo, hi) for a split task, whose
+        // rows go to split rows sbase + sidx * (last - first).
         const int clo = kind == 3 ? task.lo : 0, chi = kind == 3 ? task.hi : m;
-        for (int t0 = 0; t0 < (kind == 4 ? 0 : k); t0 += KT) {
+        const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
+        const int srow = task.sbase + sidx * (last - first) - first;
+        for (int t0 = 0; t0 < k; t0 += KT) {
             const int kt = min(KT, k - t0);
             if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
                 gather_partials(a, bv, pv, rb, (F *)nullptr, a0, first, last, t0, kt, k, KT, KT, true);
@@ -150,19 +149,17 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     F s = rv[lr * KT + t];
                     for (int g = 1; g < segs; ++g) s += rv[(g * lanes + lr) * KT + t];
                     const int col = t0 + t;
-                    if (kind == 3) {
-                        publish(&sp[(long long)(task.sbase + rho - first) * k + col], s);
-                    } else if (rho < m) {
-                        const long long o = (long long)(a0 + rho) * k + col;
-                        yv[o] = s;
-                        zv[o] = from_bits<F>(Bits<F>::sentinel);
-                    } else {
-                        publish(&pv[(long long)(a.rptr[blk] + rho - m) * k + col], s);
-                    }
+                    if (kind == 3)
+                        publish(&sp[(long long)(srow + rho) * k + col], s);
+                    else
+                        finish(rho, col, s);
                 }
             }
             __syncthreads();
         }
+        if (kind == 3 && last_split(a, task.sbase, nsp, &s_flag))
+            combine_splits(a, sp, last - first, k, task.sbase, nsp,
+                           [&](int i, int t, F v) { finish(first + i, t, v); });
         if (a.trace && tid == 0) {
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
             a.trace[4 * (size_t)tk + 3] = blockIdx.x;
@@ -181,7 +178,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
     // One chunk of v = [y_B ; z_R] (a.chunk rows x KT); after the products
     // the same space holds the 256 x KT per-thread sums.
     F *vs = reinterpret_cast<F *>(smem);
-    __shared__ int s_next;
+    __shared__ int s_next, s_flag;
     const int tid = threadIdx.x;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -217,14 +214,12 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                 x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
             }
         };
-        if (kind == 2) {   // combine the split results of columns [first, last)
-            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit,
-                           [&](int i, int t, F v) { finish(first + i, t, v); });
-            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-        }
-        // Reduction range: all rows, or [lo, hi) for a split task.
+        // Reduction range: all rows, or [lo, hi) for a split task, whose
+        // columns go to split rows sbase + sidx * (last - first).
         const int hlo = kind == 1 ? task.lo : 0, hhi = kind == 1 ? task.hi : rows;
-        for (int t0 = 0; t0 < (kind == 2 ? 0 : k); t0 += KT) {
+        const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
+        const int srow = task.sbase + sidx * (last - first) - first;
+        for (int t0 = 0; t0 < k; t0 += KT) {
             const int kt = min(KT, k - t0);
             F acc[KT];
 #pragma unroll
@@ -280,16 +275,19 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     for (int g = 1; g < segs; ++g) s += vs[(g * lanes + lr) * KT + t];
                     const int col = t0 + t;
                     if (kind == 1)
-                        publish(&sp[(long long)(task.sbase + c - first) * k + col], s);
+                        publish(&sp[(long long)(srow + c) * k + col], s);
                     else
                         finish(c, col, s);
                 }
             }
             __syncthreads();
         }
+        if (kind == 1 && last_split(a, task.sbase, nsp, &s_flag))
+            combine_splits(a, sp, last - first, k, task.sbase, nsp,
+                           [&](int i, int t, F v) { finish(first + i, t, v); });
         // Restore the sentinel in this block's partial slots for the next solve
-        // (once per block: its first column task, or that task's combine).
-        if (first == 0 && kind != 1) {
+        // (once per block: its first column task, or that task's first split).
+        if (first == 0 && sidx == 0) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
             for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
             F *rb = a.rbuf + vm.vbase;
@@ -340,6 +338,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.part = static_cast<F *>(bf.part);
     a.rbuf = static_cast<F *>(bf.rbuf);
     a.spart = static_cast<F *>(bf.spart);
+    a.scnt = bf.scnt;
     a.x = bf.x;
     a.status = bf.status;
     a.xmode = bf.xmode;

@@ -118,7 +118,8 @@ __device__ __forceinline__ int panel_chunk(int lds_bytes, int ks) {
 // so a round costs three dependent memory latencies however wide k is.
 // to_rbuf: publish the sums to rb; otherwise store them to dst[i * stride + t].
 constexpr int kStageRows = 128;
-constexpr int kStageSlots = 4;
+constexpr int kStageSlots = 16;
+constexpr int kSlotGroup = 4;   // partial loads issued together per element
 // Elements per thread per load round: 4 (f32) or 2 (f64) entries, each with
 // its kStageSlots partials, bounds the registers this phase adds.
 template <typename F> constexpr int stage_elems() { return sizeof(F) == 4 ? 4 : 2; }
@@ -151,8 +152,10 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
         __syncthreads();
         const int total = nr * k;
         for (int base = tid; base < total; base += kTaskThreads * kStageElems) {
-            U ub[kStageElems], us[kStageElems][kStageSlots];
+            U ub[kStageElems];
             int ii[kStageElems], tt[kStageElems];
+            F v[kStageElems];
+            int ncnt = 0;
 #pragma unroll
             for (int e = 0; e < kStageElems; ++e) {
                 const int idx = base + e * kTaskThreads;
@@ -160,31 +163,43 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
                 ii[e] = ok ? idx / k : 0;
                 tt[e] = ok ? idx - ii[e] * k : 0;
                 ub[e] = ok ? as_bits(bv[(long long)st.perm[ii[e]] * k + tt[e]]) : U(0);
+                if (ok) ncnt = max(ncnt, min(st.cnt[ii[e]], kStageSlots));
+            }
 #pragma unroll
-                for (int j = 0; j < kStageSlots; ++j) {
-                    const int sl = ok ? st.slot[ii[e]][j] : -1;
-                    us[e][j] = sl >= 0 ? load_bits(&pv[(long long)sl * k + tt[e]]) : U(0);
-                }
+            for (int e = 0; e < kStageElems; ++e) v[e] = from_bits<F>(ub[e]);
+            // Staged partials, kSlotGroup per element in flight at a time.
+            for (int g = 0; g < ncnt; g += kSlotGroup) {
+                U us[kStageElems][kSlotGroup];
+#pragma unroll
+                for (int e = 0; e < kStageElems; ++e)
+#pragma unroll
+                    for (int j = 0; j < kSlotGroup; ++j) {
+                        const int idx = base + e * kTaskThreads;
+                        const int sl = idx < total ? st.slot[ii[e]][g + j] : -1;
+                        us[e][j] = sl >= 0 ? load_bits(&pv[(long long)sl * k + tt[e]]) : U(0);
+                    }
+#pragma unroll
+                for (int e = 0; e < kStageElems; ++e)
+#pragma unroll
+                    for (int j = 0; j < kSlotGroup; ++j) {
+                        const int idx = base + e * kTaskThreads;
+                        const int sl = idx < total ? st.slot[ii[e]][g + j] : -1;
+                        if (sl >= 0) v[e] += ready_value(&pv[(long long)sl * k + tt[e]], us[e][j], a.status);
+                    }
             }
 #pragma unroll
             for (int e = 0; e < kStageElems; ++e) {
                 const int idx = base + e * kTaskThreads;
                 if (idx >= total) break;
                 const int i = ii[e], t = tt[e];
-                F v = from_bits<F>(ub[e]);
-#pragma unroll
-                for (int j = 0; j < kStageSlots; ++j) {
-                    const int sl = st.slot[i][j];
-                    if (sl >= 0) v += ready_value(&pv[(long long)sl * k + t], us[e][j], a.status);
-                }
-                for (int j = kStageSlots; j < st.cnt[i]; ++j) {   // rows with many contributions
+                for (int j = kStageSlots; j < st.cnt[i]; ++j) {   // rows with very many contributions
                     const F *p = &pv[(long long)a.cslot[st.first[i] + j] * k + t];
-                    v += ready_value(p, load_bits(p), a.status);
+                    v[e] += ready_value(p, load_bits(p), a.status);
                 }
                 if (to_rbuf)
-                    publish(&rb[(long long)(a0 + q0 + i) * k + t], v);
+                    publish(&rb[(long long)(a0 + q0 + i) * k + t], v[e]);
                 else
-                    dst[(q0 - r0 + i) * stride + t] = v;
+                    dst[(q0 - r0 + i) * stride + t] = v[e];
             }
         }
         __syncthreads();
@@ -217,7 +232,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
     using Acc = typename Mfma<F>::Acc;
     extern __shared__ __align__(16) unsigned char smem[];
     F *rv = reinterpret_cast<F *>(smem);
-    __shared__ int s_next;
+    __shared__ int s_next, s_flag;
     __shared__ GatherStage stage;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int qid = blockIdx.x % a.qa.nq;
@@ -237,20 +252,19 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
         F *sp = a.spart + vm.sbase;
         const F *bv = a.b + vm.vbase;
-        if (kind == 4) {   // combine the split results of rows [first, last)
-            const long long sbase = a.rptr[blk];
-            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit, [&](int i, int t, F v) {
-                const int r = first + i;
-                if (r < m) {
-                    const long long o = (long long)(a0 + r) * k + t;
-                    yv[o] = v;
-                    zv[o] = from_bits<F>(Bits<F>::sentinel);
-                } else {
-                    publish(&pv[(sbase + r - m) * k + t], v);
-                }
-            });
-            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-        } else if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
+        // Finishes row r, column t of the task: y (rows < m) or a partial slot.
+        auto finish = [&](int r, int t, F v) {
+            if (r < m) {
+                const long long o = (long long)(a0 + r) * k + t;
+                yv[o] = v;
+                zv[o] = from_bits<F>(Bits<F>::sentinel);
+            } else {
+                publish(&pv[(long long)(a.rptr[blk] + r - m) * k + t], v);
+            }
+        };
+        const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
+        const int srow = task.sbase + sidx * (last - first) - first;
+        if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
             gather_wide(a, stage, bv, pv, rb, (F *)nullptr, a0, first, last, k, k, true);
             if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
         } else if (k > 0) {
@@ -345,7 +359,6 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
             }
             if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
             if (w.ntiles > 0 && w.split == 0) {
-                const long long sbase = a.rptr[blk];
 #pragma unroll
                 for (int j = 0; j < kMaxTiles; ++j) {
                     if (j >= w.ntiles) break;
@@ -356,18 +369,16 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
                         const int rho = g0 + Mfma<F>::row(kk, v);
                         if (rho >= last) break;
                         const F s = acc[j][v];
-                        if (kind == 3) {
-                            publish(&sp[(long long)(task.sbase + rho - first) * k + t], s);
-                        } else if (rho < m) {
-                            const long long o = (long long)(a0 + rho) * k + t;
-                            yv[o] = s;
-                            zv[o] = from_bits<F>(Bits<F>::sentinel);
-                        } else {
-                            publish(&pv[(sbase + rho - m) * k + t], s);
-                        }
+                        if (kind == 3)
+                            publish(&sp[(long long)(srow + rho) * k + t], s);
+                        else
+                            finish(rho, t, s);
                     }
                 }
             }
+            if (kind == 3 && last_split(a, task.sbase, nsp, &s_flag))
+                combine_splits(a, sp, last - first, k, task.sbase, nsp,
+                               [&](int i, int t, F v) { finish(first + i, t, v); });
         }
         if (a.trace && tid == 0) {
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
@@ -386,7 +397,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
     using Acc = typename Mfma<F>::Acc;
     extern __shared__ __align__(16) unsigned char smem[];
     F *vs = reinterpret_cast<F *>(smem);
-    __shared__ int s_next;
+    __shared__ int s_next, s_flag;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -417,11 +428,9 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                 x64[o] = (a.xmode == 2 ? x64[o] : 0.0) + (double)s;
             }
         };
-        if (kind == 2) {   // combine the split results of columns [first, last)
-            combine_splits(a, sp, last - first, k, task.sbase, task.nsplit,
-                           [&](int i, int t, F v) { finish(first + i, t, v); });
-            if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-        } else if (k > 0) {
+        const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
+        const int srow = task.sbase + sidx * (last - first) - first;
+        if (k > 0) {
             const int nT = (k + 15) / 16;
             const WaveWork w = wave_work(wave, last - first, nT);
             const int cg0 = first + 16 * max(w.group, 0);
@@ -495,16 +504,19 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                         if (cc >= last) break;
                         const F s = acc[j][v];
                         if (kind == 1)
-                            publish(&sp[(long long)(task.sbase + cc - first) * k + t], s);
+                            publish(&sp[(long long)(srow + cc) * k + t], s);
                         else
                             finish(cc, t, s);
                     }
                 }
             }
+            if (kind == 1 && last_split(a, task.sbase, nsp, &s_flag))
+                combine_splits(a, sp, last - first, k, task.sbase, nsp,
+                               [&](int i, int t, F v) { finish(first + i, t, v); });
         }
         // Restore the sentinel in this block's partial slots for the next solve
-        // (once per block: its first column task, or that task's combine).
-        if (first == 0 && kind != 1) {
+        // (once per block: its first column task, or that task's first split).
+        if (first == 0 && sidx == 0) {
             const F sent = from_bits<F>(Bits<F>::sentinel);
             for (int idx = tid; idx < R * k; idx += kTaskThreads) pv[(long long)r0 * k + idx] = sent;
             F *rb = a.rbuf + vm.vbase;
