@@ -27,7 +27,7 @@ constexpr int split_index(int code) { return code >> 8; }
 
 struct BlockPlanOptions {
     int max_block = 1024;   // rows per block (dense inverse is m x m)
-    int leaf_max = 32;      // whole subtrees up to this size become one block
+    int leaf_max = 64;      // whole subtrees up to this size become one block
     int fwd_rows = 64;      // rows of M_B per forward task
     int bwd_cols = 32;      // columns of M_B per backward task
     int split_gather = 65;  // blocks with m >= this gather r_B in separate tasks
