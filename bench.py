@@ -102,7 +102,7 @@ def main():
     ap.add_argument("--level", type=int, default=7, help="icosphere subdivision level")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--workload", choices=["smoothing", "batch", "precision"], default="smoothing")
+    ap.add_argument("--workload", choices=["smoothing", "batch", "precision", "grad"], default="smoothing")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--profile", action="store_true",
                     help="headline path only, then a 256 MiB copy (for rocprofv3 counter passes)")
@@ -115,6 +115,8 @@ def main():
         return bench_batch(args)
     if args.workload == "precision":
         return bench_precision(args)
+    if args.workload == "grad":
+        return bench_grad(args)
 
     import torch
 
@@ -446,6 +448,55 @@ def bench_precision(args):
         },
     }
     print(json.dumps(out))
+
+
+def bench_grad(args):
+    """Differentiable solve on icosphere(6) (n = 40,962), A = I + 10 (D - W),
+    3 right-hand sides, FP32 solver: loss = ||x - t||^2 and its gradients
+    with respect to b and every stored entry of A (adjoint solve + HIP
+    scatter). A step is one forward solve + backward; the FP32 gradients
+    are compared with the FP64 solver's."""
+    import torch
+
+    from cholespy_amd import CholeskySolverD, CholeskySolverF, MatrixType, workloads
+    from cholespy_amd.autograd import coo_indices, solve
+
+    dev = torch.device("cuda:0")
+    n, faces = workloads.icosphere(6)
+    values, idx = workloads.mesh_laplacian(n, faces, 10.0)
+    rng = np.random.default_rng(0)
+    b = rng.standard_normal((n, 3))
+    t = rng.standard_normal((n, 3))
+    ii, jj = torch.tensor(idx[0], device=dev), torch.tensor(idx[1], device=dev)
+    rows, cols = coo_indices(n, ii, jj, MatrixType.COO)
+    out = {}
+    for name, cls, dt in (("f32", CholeskySolverF, torch.float32), ("f64", CholeskySolverD, torch.float64)):
+        vals = torch.tensor(values, device=dev, requires_grad=True)
+        solver = cls(n, ii, jj, vals.detach(), MatrixType.COO)
+        bt = torch.tensor(b, device=dev, dtype=dt, requires_grad=True)
+        tt = torch.tensor(t, device=dev, dtype=dt)
+
+        def step():
+            vals.grad = None
+            bt.grad = None
+            x = solve(solver, bt, vals, rows, cols)
+            ((x - tt) ** 2).sum().backward()
+
+        ms = time_events(step, max(20, args.steps // 4))
+        solver._check()
+        out[name] = {"ms": ms, "gv": vals.grad.double().cpu().numpy(), "gb": bt.grad.double().cpu().numpy()}
+        log(f"{name}: {ms:.3f} ms per forward + backward")
+    rel = float(np.abs(out["f32"]["gv"] - out["f64"]["gv"]).max() / np.abs(out["f64"]["gv"]).max())
+    ms = out["f32"]["ms"]
+    print(json.dumps({
+        "metric": METRIC, "value": 3 * 1000.0 / ms, "unit": "RHS-cols/s (forward + backward)",
+        "n_gpus": 1, "steps": max(20, args.steps // 4), "warmup": 3, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "differentiable solve, icosphere(6), lambda=10, k=3, loss ||x - t||^2",
+                   "n": n, "ms_f32": ms, "ms_f64": out["f64"]["ms"],
+                   "entry_grad_rel_diff_f32_vs_f64": rel},
+    }))
 
 
 def bench_batch(args):

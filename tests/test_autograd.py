@@ -82,7 +82,7 @@ def test_gradient_vs_central_differences(device):
     """>= 200 seeded stored entries (diagonal, below and above the diagonal,
     duplicates included) against FP64 central differences of the loss
     evaluated with SciPy's spsolve; step h = 1e-6 * max(1, |v_e|)."""
-    n, values, idx, b, t = system(3)
+    n, values, idx, b, t = system(4)
     ii = torch.tensor(idx[0], device=device)
     jj = torch.tensor(idx[1], device=device)
     vals = torch.tensor(values, device=device, requires_grad=True)
@@ -108,6 +108,36 @@ def test_gradient_vs_central_differences(device):
         vm[e] -= h
         fd = (loss(vp) - loss(vm)) / (2 * h)
         assert abs(fd - g[e]) <= 1e-6 * max(1.0, abs(g[e])) + 1e-7, (e, fd, g[e])
+
+
+@pytest.mark.gpu
+def test_full_size_gradient_fp32_vs_fp64():
+    """The differentiation workload at full size: icosphere(6) (n = 40,962),
+    lambda = 10, 3 right-hand sides, loss ||x - t||^2. The FP32 solver's
+    gradients (b and every stored entry, HIP scatter) agree with the FP64
+    solver's to FP32 accuracy, and the FP64 gradient of b is the adjoint
+    solution (residual of A lam = 2 (x - t) at FP64 level)."""
+    n, values, idx, b, t = system(6)
+    ii = torch.tensor(idx[0], device="cuda")
+    jj = torch.tensor(idx[1], device="cuda")
+    rows, cols = coo_indices(n, ii, jj, MatrixType.COO)
+    grads = {}
+    for cls, dt in ((CholeskySolverF, torch.float32), (CholeskySolverD, torch.float64)):
+        vals = torch.tensor(values, device="cuda", requires_grad=True)
+        solver = cls(n, ii, jj, vals.detach(), MatrixType.COO)
+        bt = torch.tensor(b, device="cuda", dtype=dt, requires_grad=True)
+        x = solve(solver, bt, vals, rows, cols)
+        ((x - torch.tensor(t, device="cuda", dtype=dt)) ** 2).sum().backward()
+        torch.cuda.synchronize()
+        solver._check()
+        grads[dt] = (vals.grad.double().cpu().numpy(), bt.grad.double().cpu().numpy(), x.detach().double().cpu().numpy())
+    gv32, gb32, _ = grads[torch.float32]
+    gv64, gb64, x64 = grads[torch.float64]
+    assert np.abs(gv32 - gv64).max() / np.abs(gv64).max() < 1e-4
+    assert np.abs(gb32 - gb64).max() / np.abs(gb64).max() < 1e-4
+    A = sp.coo_matrix((values, (idx[0], idx[1])), shape=(n, n)).tocsr()
+    r = A @ gb64 - 2.0 * (x64 - t)
+    assert np.linalg.norm(r) / np.linalg.norm(2.0 * (x64 - t)) < 1e-12
 
 
 def test_csr_and_csc_indices():
