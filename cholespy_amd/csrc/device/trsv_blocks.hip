@@ -10,9 +10,10 @@
 //   z published (waited on by descendants), x[perm[a+c]] = z_B[c].
 // No task waits on a task of its own block, and every wait targets a block of
 // a strictly lower block level, i.e. an earlier ticket; workgroups take
-// tickets in order from the queues of common.hpp (the next one prefetched
-// while the current task runs), so every awaited task is running or done and
-// the schedule cannot deadlock.
+// tickets in order from the queues of common.hpp when they start a task (not
+// earlier: a ticket taken ahead would hold a ready task back behind the
+// current one), so every awaited task is running or done and the schedule
+// cannot deadlock.
 //
 // Each task is one workgroup of 256 threads: up to 64 rows x 4 column
 // segments (forward; a wavefront loads 64 consecutive entries of one column,
@@ -51,8 +52,6 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
     while (tk >= 0) {
-        int pre = 0;
-        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
         const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
@@ -164,7 +163,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
             a.trace[4 * (size_t)tk + 3] = blockIdx.x;
         }
-        if (tid == 0) s_next = pre;
+        if (tid == 0) s_next = take_ticket(a.qa, qid, a.ntasks);
         __syncthreads();
         tk = s_next;
         __syncthreads();
@@ -183,8 +182,6 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
     while (tk >= 0) {
-        int pre = 0;
-        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
         const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
@@ -297,7 +294,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
             a.trace[4 * (size_t)tk + 3] = blockIdx.x;
         }
-        if (tid == 0) s_next = pre;
+        if (tid == 0) s_next = take_ticket(a.qa, qid, a.ntasks);
         __syncthreads();
         tk = s_next;
         __syncthreads();

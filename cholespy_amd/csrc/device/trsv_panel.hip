@@ -238,8 +238,6 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
     while (tk >= 0) {
-        int pre = 0;
-        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
         const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
@@ -384,7 +382,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
             a.trace[4 * (size_t)tk + 3] = blockIdx.x;
         }
-        if (tid == 0) s_next = pre;
+        if (tid == 0) s_next = take_ticket(a.qa, qid, a.ntasks);
         __syncthreads();
         tk = s_next;
         __syncthreads();
@@ -402,8 +400,6 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
     while (tk >= 0) {
-        int pre = 0;
-        if (tid == 0) pre = take_ticket(a.qa, qid, a.ntasks);   // in flight during the task
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
         const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
@@ -526,7 +522,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
             a.trace[4 * (size_t)tk + 3] = blockIdx.x;
         }
-        if (tid == 0) s_next = pre;
+        if (tid == 0) s_next = take_ticket(a.qa, qid, a.ntasks);
         __syncthreads();
         tk = s_next;
         __syncthreads();
