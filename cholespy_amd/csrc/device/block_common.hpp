@@ -58,10 +58,11 @@ struct Task {
 };
 
 // Panel (MFMA) variants of the block passes, trsv_panel.hip.
+// nt = right-hand-side tiles of 16 the launch holds (2, 4 or 8).
 int panel_lds_bytes();
-const void *block_panel_kernel(int float_bytes, bool forward);
-hipError_t launch_panel_f32(bool forward, const BlockPassArgs<float> &a, int grid, hipStream_t stream);
-hipError_t launch_panel_f64(bool forward, const BlockPassArgs<double> &a, int grid, hipStream_t stream);
+const void *block_panel_kernel(int float_bytes, bool forward, int nt);
+hipError_t launch_panel_f32(bool forward, const BlockPassArgs<float> &a, int nt, int grid, hipStream_t stream);
+hipError_t launch_panel_f64(bool forward, const BlockPassArgs<double> &a, int nt, int grid, hipStream_t stream);
 
 namespace {
 

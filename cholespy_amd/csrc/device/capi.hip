@@ -132,7 +132,7 @@ struct cholespy_gpu_solver {
     float *r32 = nullptr;                 // FP32 right-hand side of correction solves
     size_t r32_elems = 0;
     int grid_rows[2][2] = {{0, 0}, {0, 0}};   // [lower][k > 64]
-    int grid_blocks[2][2] = {{0, 0}, {0, 0}};   // [forward][wide tile]
+    int grid_blocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // [forward][tile_slot]
     // Ticket queues per kernel: [0..1] analysis upper/lower, [2..3] row pass
     // backward/forward, [4..5] block pass backward/forward.
     unsigned int *qctr = nullptr;
@@ -222,7 +222,8 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
         HIP_TRY(upload(reinterpret_cast<double **>(&d.mfwd), P.mfwd.data(), me));
         HIP_TRY(upload(reinterpret_cast<double **>(&d.mbwd), P.mbwd.data(), me));
     }
-    s->grid_blocks[0][0] = s->grid_blocks[0][1] = s->grid_blocks[1][0] = s->grid_blocks[1][1] = 0;
+    for (auto &row : s->grid_blocks)
+        for (int &g : row) g = 0;
     s->buffers_path = -1;
     return CHOLESPY_OK;
 }
@@ -287,6 +288,9 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
     if (e == hipSuccess) e = hipMemset(s->qctr, 0, qwords * sizeof(unsigned int));
     if (e == hipSuccess)
         for (int i = 0; i < 6; ++i) s->queues[i].ctr = s->qctr + (size_t)i * kQueueWords;
+        // Block-pass ticket queues (tuning sweeps: CHOLESPY_BLOCK_QUEUES, 1..16).
+        if (const char *e = std::getenv("CHOLESPY_BLOCK_QUEUES"))
+            s->queues[4].nq = s->queues[5].nq = std::min(16, std::max(1, std::atoi(e)));
     if (e != hipSuccess) {
         cholespy_solver_destroy(s);
         return hip_fail(e, "cholespy_solver_create");
@@ -617,7 +621,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
         bf.M = forward ? d.mfwd : d.mbwd;
         bf.trace = s->trace[forward ? 1 : 0];
         const int tile = block_tile(fbytes, bf.max_k);
-        int &grid = s->grid_blocks[forward ? 1 : 0][tile > 4 ? 1 : 0];
+        int &grid = s->grid_blocks[forward ? 1 : 0][tile_slot(tile)];
         if (grid == 0) {
             const size_t lds = block_pass_lds(fbytes, forward ? d.max_m : d.max_rows, tile);
             const int cap = max_coresident_blocks(block_pass_kernel(fbytes, forward, tile), 256, lds);

@@ -53,14 +53,23 @@ hipError_t launch_row_pass(int float_bytes, bool lower, const RowPassBuffers &bu
 // Partitioned-inverse block passes (block_plan.hpp / trsv_blocks.hip,
 // trsv_panel.hip). Up to 4 right-hand sides: vector-FMA kernels with one
 // 4-column tile; more: the MFMA panel kernels (all columns in one sweep).
+// block_tile returns kNarrowTile, or the right-hand-side columns (32, 64 or
+// 128, i.e. 2, 4 or 8 tiles of 16) the panel kernel is instantiated for.
 constexpr int kNarrowTile = 4;
-constexpr int kPanelTile = 16;
-inline int block_tile(int float_bytes, int k) { (void)float_bytes; return k <= kNarrowTile ? kNarrowTile : kPanelTile; }
+inline int block_tile(int float_bytes, int k) {
+    (void)float_bytes;
+    if (k <= kNarrowTile) return kNarrowTile;
+    return k <= 32 ? 32 : (k <= 64 ? 64 : 128);
+}
+inline bool is_panel_tile(int tile) { return tile > kNarrowTile; }
+// Index of a tile choice in small per-handle caches (0 narrow, 1..3 panel).
+inline int tile_slot(int tile) { return tile <= kNarrowTile ? 0 : (tile <= 32 ? 1 : (tile <= 64 ? 2 : 3)); }
 // Rows of a vector a narrow block task stages in LDS at a time: a 32 KiB
 // chunk, so four or five workgroups fit a CU's 160 KiB of LDS.
 constexpr int kBlockLdsBytes = 32 * 1024;
 inline int block_chunk(int float_bytes, int tile) { return kBlockLdsBytes / (tile * float_bytes); }
-// Dynamic LDS of a panel launch (override: CHOLESPY_PANEL_LDS, 16-64 KiB).
+// Dynamic LDS of a panel launch (override: CHOLESPY_PANEL_LDS, 16-64 KiB);
+// at least 16 KiB, the backward kernel's staged R-row indices.
 constexpr int kPanelLdsBytes = 32 * 1024;
 struct BlockPassBuffers {
     int ntasks = 0, k = 0;

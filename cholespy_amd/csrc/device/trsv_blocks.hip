@@ -346,9 +346,9 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.sys_pbase = bf.sys_pbase;
     a.sys_sbase = bf.sys_sbase;
     queues.fill(a.qa);
-    if (tile == kPanelTile) {
-        if constexpr (sizeof(F) == 4) return launch_panel_f32(forward, a, grid, stream);
-        else return launch_panel_f64(forward, a, grid, stream);
+    if (is_panel_tile(tile)) {
+        if constexpr (sizeof(F) == 4) return launch_panel_f32(forward, a, tile / 16, grid, stream);
+        else return launch_panel_f64(forward, a, tile / 16, grid, stream);
     }
     const size_t lds = block_pass_lds(sizeof(F), forward ? bf.max_m : bf.max_rows, tile);
     return launch_narrow<F>(forward, a, tile, lds, grid, stream);
@@ -357,13 +357,13 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
 }  // namespace
 
 size_t block_pass_lds(int float_bytes, int need_rows, int tile) {
-    if (tile == kPanelTile) return (size_t)panel_lds_bytes();
+    if (is_panel_tile(tile)) return (size_t)panel_lds_bytes();
     const int rows = std::max(std::min(need_rows, block_chunk(float_bytes, tile)), kTaskThreads);
     return (size_t)rows * tile * (size_t)float_bytes;
 }
 
 const void *block_pass_kernel(int float_bytes, bool forward, int tile) {
-    if (tile == kPanelTile) return block_panel_kernel(float_bytes, forward);
+    if (is_panel_tile(tile)) return block_panel_kernel(float_bytes, forward, tile / 16);
     if (float_bytes == 4)
         return forward ? reinterpret_cast<const void *>(&block_forward<float, kNarrowTile>)
                        : reinterpret_cast<const void *>(&block_backward<float, kNarrowTile>);

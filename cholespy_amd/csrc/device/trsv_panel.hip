@@ -52,8 +52,13 @@ template <> struct Mfma<double> {
     static __device__ __forceinline__ int row(int q, int v) { return q + 4 * v; }
 };
 
-constexpr int kMaxTiles = kMaxRhs / 16;
-constexpr int kSteps = 4;   // MFMA steps whose A loads are issued together
+// Kernels are instantiated for NT = 2, 4 or 8 tiles of 16 right-hand sides
+// (the launch picks the smallest that holds the largest k), so small k keeps
+// few accumulators and loads more steps ahead: kSteps(NT) MFMA steps have
+// their A (and B) loads issued together.
+template <int NT> constexpr int kStepsFor() { return NT <= 2 ? 8 : 4; }
+// Backward tasks stage the R-row indices they read in LDS (up to this many).
+constexpr int kStageRlist = 4096;
 
 // Packed offsets of M_B entries (32-bit: one block's M_B has < 2^31 entries).
 __device__ __forceinline__ int fwd_off32(int rows, int c, int rho) {
@@ -227,9 +232,11 @@ __device__ __forceinline__ void reduce_splits(F *red, const WaveWork &w, int wav
         }
 }
 
-template <typename F>
+template <typename F, int NT>
 __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArgs<F> a) {
     using Acc = typename Mfma<F>::Acc;
+    constexpr int kMaxTiles = NT;
+    constexpr int kSteps = kStepsFor<NT>();
     extern __shared__ __align__(16) unsigned char smem[];
     F *rv = reinterpret_cast<F *>(smem);
     __shared__ int s_next, s_flag;
@@ -390,9 +397,11 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
     if (tid == 0) finish_queues(a.qa);
 }
 
-template <typename F>
+template <typename F, int NT>
 __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassArgs<F> a) {
     using Acc = typename Mfma<F>::Acc;
+    constexpr int kMaxTiles = NT;
+    constexpr int kSteps = kStepsFor<NT>();
     extern __shared__ __align__(16) unsigned char smem[];
     F *vs = reinterpret_cast<F *>(smem);
     __shared__ int s_next, s_flag;
@@ -435,6 +444,16 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
             Acc acc[kMaxTiles];
 #pragma unroll
             for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
+            // The R rows the task reads, [rb0, rb1), have their indices staged
+            // in LDS (one load each, all in parallel) when they fit.
+            const int he = kind == 1 ? task.hi : rows;
+            const int rb0 = max(m, kind == 1 ? max(first, task.lo) : first);
+            const bool staged = he - rb0 <= kStageRlist;
+            int *srl = reinterpret_cast<int *>(vs);
+            __syncthreads();   // the previous task's reads of vs are done
+            if (staged)
+                for (int r = rb0 + tid; r < he; r += kTaskThreads) srl[r - rb0] = a.rlist[r0 + r - m];
+            __syncthreads();
             if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
             if (w.ntiles > 0) {
                 // B = v[rho] = y_B[rho] (rho < m, complete since the forward
@@ -443,7 +462,6 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                 // first column are zero in its columns.
                 using U = typename Bits<F>::U;
                 const int hs = kind == 1 ? max(cg0, task.lo) : cg0;   // rows [hs, he)
-                const int he = kind == 1 ? task.hi : rows;
                 const int nsteps = (he - hs + 3) / 4;
                 const int toff = (lane & 15) + 16 * w.tile0;
                 for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
@@ -459,7 +477,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                         bp[q] = nullptr;
                         if (s < nsteps && rho < he)
                             bp[q] = (rho < m ? yv + (long long)(a0 + rho) * k
-                                             : zv + (long long)a.rlist[r0 + rho - m] * k) + toff;
+                                             : zv + (long long)(staged ? srl[rho - rb0] : a.rlist[r0 + rho - m]) * k) +
+                                    toff;
                     }
                     U bu[kSteps][kMaxTiles];
 #pragma unroll
@@ -487,7 +506,10 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                     }
                 }
             }
-            if (w.nsplit > 1) reduce_splits(vs, w, wave, lane, acc[0]);
+            if (w.nsplit > 1) {
+                __syncthreads();   // the staged R-row indices are no longer read
+                reduce_splits(vs, w, wave, lane, acc[0]);
+            }
             if (w.ntiles > 0 && w.split == 0) {
 #pragma unroll
                 for (int j = 0; j < kMaxTiles; ++j) {
@@ -541,28 +563,29 @@ int panel_lds_bytes() {
     return bytes;
 }
 
-const void *block_panel_kernel(int float_bytes, bool forward) {
-    if (float_bytes == 4)
-        return forward ? reinterpret_cast<const void *>(&block_forward_panel<float>)
-                       : reinterpret_cast<const void *>(&block_backward_panel<float>);
-    return forward ? reinterpret_cast<const void *>(&block_forward_panel<double>)
-                   : reinterpret_cast<const void *>(&block_backward_panel<double>);
+const void *block_panel_kernel(int float_bytes, bool forward, int nt) {
+#define CHOLESPY_PANEL(F, NT)                                                                 \
+    (forward ? reinterpret_cast<const void *>(&block_forward_panel<F, NT>)                   \
+             : reinterpret_cast<const void *>(&block_backward_panel<F, NT>))
+    if (float_bytes == 4) return nt <= 2 ? CHOLESPY_PANEL(float, 2) : nt <= 4 ? CHOLESPY_PANEL(float, 4) : CHOLESPY_PANEL(float, 8);
+    return nt <= 2 ? CHOLESPY_PANEL(double, 2) : nt <= 4 ? CHOLESPY_PANEL(double, 4) : CHOLESPY_PANEL(double, 8);
+#undef CHOLESPY_PANEL
 }
 
 template <typename F>
-static hipError_t launch_panel_impl(bool forward, const BlockPassArgs<F> &args, int grid, hipStream_t stream) {
+static hipError_t launch_panel_impl(bool forward, const BlockPassArgs<F> &args, int nt, int grid, hipStream_t stream) {
     BlockPassArgs<F> a = args;
     a.lds_bytes = panel_lds_bytes();
     void *p[] = {&a};
-    return hipLaunchKernel(block_panel_kernel(sizeof(F), forward), dim3(grid), dim3(kTaskThreads), p,
+    return hipLaunchKernel(block_panel_kernel(sizeof(F), forward, nt), dim3(grid), dim3(kTaskThreads), p,
                            (size_t)a.lds_bytes, stream);
 }
 
-hipError_t launch_panel_f32(bool forward, const BlockPassArgs<float> &a, int grid, hipStream_t stream) {
-    return launch_panel_impl<float>(forward, a, grid, stream);
+hipError_t launch_panel_f32(bool forward, const BlockPassArgs<float> &a, int nt, int grid, hipStream_t stream) {
+    return launch_panel_impl<float>(forward, a, nt, grid, stream);
 }
-hipError_t launch_panel_f64(bool forward, const BlockPassArgs<double> &a, int grid, hipStream_t stream) {
-    return launch_panel_impl<double>(forward, a, grid, stream);
+hipError_t launch_panel_f64(bool forward, const BlockPassArgs<double> &a, int nt, int grid, hipStream_t stream) {
+    return launch_panel_impl<double>(forward, a, nt, grid, stream);
 }
 
 }  // namespace cholespy_amd
