@@ -1,5 +1,7 @@
 #include "block_plan.hpp"
 
+#include "../host/dense.hpp"
+
 #include <atomic>
 #include <functional>
 #include <thread>
@@ -105,44 +107,29 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
             const int R = P.rptr[b + 1] - P.rptr[b];
             const int *rl = P.rlist.data() + P.rptr[b];
             for (int q = 0; q < R; ++q) pos[rl[q]] = q;
-            Lbb.assign((size_t)m * m, 0.0);   // row-major
-            Pb.assign((size_t)R * m, 0.0);    // row-major
+            // Column-major L_BB (m x m) and L_RB (R x m); X := I L_BB^{-T}
+            // = L_BB^{-T} (so L_BB^{-1}[rho][c] = X[c][rho]); then
+            // C = -L_RB X^T = -L_RB L_BB^{-1}, the lower part of M_B.
+            Lbb.assign((size_t)m * m, 0.0);
+            Pb.assign((size_t)R * m, 0.0);
             for (int c = a; c < e; ++c)
                 for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
                     const int i = rowidx[p];
-                    if (i < e) Lbb[(size_t)(i - a) * m + (c - a)] = val[p];
-                    else Pb[(size_t)pos[i] * m + (c - a)] = val[p];
+                    if (i < e) Lbb[(size_t)(c - a) * m + (i - a)] = val[p];
+                    else Pb[(size_t)(c - a) * R + pos[i]] = val[p];
                 }
-            // Linv = Lbb^{-1} by forward substitution on the identity, column by column.
             Linv.assign((size_t)m * m, 0.0);
-            for (int col = 0; col < m; ++col) {
-                Linv[(size_t)col * m + col] = 1.0 / Lbb[(size_t)col * m + col];
-                for (int i = col + 1; i < m; ++i) {
-                    double s = 0.0;
-                    const double *li = &Lbb[(size_t)i * m];
-                    for (int kk = col; kk < i; ++kk) s += li[kk] * Linv[(size_t)kk * m + col];
-                    Linv[(size_t)i * m + col] = -s / li[i];
-                }
-            }
-            // Q = Pb * Linv (R x m); M_B = [Linv ; -Q].
+            for (int c = 0; c < m; ++c) Linv[(size_t)c * m + c] = 1.0;
+            dense::trsm_right_lower_trans(m, m, Lbb.data(), m, Linv.data(), m);
             Q.assign((size_t)R * m, 0.0);
-            for (int q = 0; q < R; ++q) {
-                const double *pr = &Pb[(size_t)q * m];
-                double *qr = &Q[(size_t)q * m];
-                for (int kk = 0; kk < m; ++kk) {
-                    const double pk = pr[kk];
-                    if (pk == 0.0) continue;
-                    const double *lr = &Linv[(size_t)kk * m];
-                    for (int c = 0; c <= kk; ++c) qr[c] += pk * lr[c];
-                }
-            }
+            if (R > 0) dense::gemm_nt()(false, R, m, m, Pb.data(), R, Linv.data(), m, Q.data(), R);
             const int64_t rows = (int64_t)m + R;
             double *F = P.mfwd.data() + P.moff[b];
             double *Bk = P.mbwd.data() + P.moff[b];
             for (int rho = 0; rho < rows; ++rho) {
                 const int cmax = rho < m ? rho : m - 1;
                 for (int c = 0; c <= cmax; ++c) {
-                    const double v = rho < m ? Linv[(size_t)rho * m + c] : -Q[(size_t)(rho - m) * m + c];
+                    const double v = rho < m ? Linv[(size_t)rho * m + c] : Q[(size_t)c * R + (rho - m)];
                     F[fwd_off(rows, c, rho)] = v;
                     Bk[bwd_off(m, rho, c)] = v;
                 }

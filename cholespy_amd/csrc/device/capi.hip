@@ -641,8 +641,9 @@ int solve_impl(cholespy_gpu_solver *s, int fbytes, int k, const void *b, void *x
         return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "solver precision does not match the call");
     if (k > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
     if (k < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative number of RHS");
-    if (!s->analysed[0] || !s->analysed[1])
-        return fail(CHOLESPY_ERR_STATE, "solve called before both passes were analysed");
+    if (s->path == CHOLESPY_PATH_ROWS && (!s->analysed[0] || !s->analysed[1]))
+        return fail(CHOLESPY_ERR_STATE, "the row schedule needs both passes analysed");
+    if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "solve called before the factor was uploaded");
     if (k == 0 || s->n == 0) return CHOLESPY_OK;
     if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
     hipStream_t st = static_cast<hipStream_t>(stream);

@@ -25,14 +25,14 @@ CholeskySolver<Float>::CholeskySolver(int n_rows, int64_t nnz, const int *ii, co
     if (cpu_) return;
 
     // GPU tail of the reference's factorize() (src/cholesky_solver.cpp:244-297):
-    // upload perm and L, then analyse L and L'.
+    // upload perm and L (and the block plan built from it). The level
+    // analysis of L and L' runs on first use (analyse()): the default block
+    // schedule does not need it.
     throw_on_error(cholespy_solver_create(n_, (int)sizeof(Float), &gpu_));
     try {
         throw_on_error(cholespy_solver_upload_factor(gpu_, factor_.perm.data(), factor_.colptr.data(),
                                                      factor_.rowidx.data(), factor_.val.data(),
                                                      factor_.nnz()));
-        throw_on_error(cholespy_solver_analyze(gpu_, 1, &n_levels_lower_));
-        throw_on_error(cholespy_solver_analyze(gpu_, 0, &n_levels_upper_));
         throw_on_error(cholespy_solver_upload_matrix(gpu_, A_.colptr.data(), A_.rowidx.data(),
                                                      A_.val.data(), A_.nnz()));
     } catch (...) {
@@ -40,6 +40,14 @@ CholeskySolver<Float>::CholeskySolver(int n_rows, int64_t nnz, const int *ii, co
         gpu_ = nullptr;
         throw;
     }
+}
+
+template <typename Float>
+void CholeskySolver<Float>::analyse() const {
+    if (analysed_ || !gpu_) return;
+    throw_on_error(cholespy_solver_analyze(gpu_, 1, &n_levels_lower_));
+    throw_on_error(cholespy_solver_analyze(gpu_, 0, &n_levels_upper_));
+    analysed_ = true;
 }
 
 template <typename Float>
@@ -74,6 +82,7 @@ template <typename Float>
 void CholeskySolver<Float>::levels(bool lower, std::vector<int> &order,
                                    std::vector<int> &row_level) const {
     if (!gpu_) throw std::invalid_argument("Level analysis only exists for GPU solvers.");
+    analyse();
     order.resize((size_t)n_);
     row_level.resize((size_t)n_);
     throw_on_error(cholespy_solver_get_levels(gpu_, lower ? 1 : 0, order.data(), row_level.data()));
@@ -101,6 +110,7 @@ void CholeskySolver<Float>::solve_refined(int n_rhs, const double *b, double *x,
 template <typename Float>
 void CholeskySolver<Float>::set_path(int path) {
     if (!gpu_) throw std::invalid_argument("Solve paths only exist for GPU solvers.");
+    if (path == CHOLESPY_PATH_ROWS) analyse();
     throw_on_error(cholespy_solver_set_path(gpu_, path));
 }
 

@@ -35,7 +35,10 @@ public:
     int n() const { return n_; }
     const Factor &factor() const { return factor_; }
     const LowerCSC &matrix() const { return A_; }
-    int n_levels(bool lower) const { return lower ? n_levels_lower_ : n_levels_upper_; }
+    int n_levels(bool lower) const {
+        analyse();
+        return lower ? n_levels_lower_ : n_levels_upper_;
+    }
     // Level-sorted rows and per-row levels as produced by the GPU analysis.
     void levels(bool lower, std::vector<int> &order, std::vector<int> &row_level) const;
     // Waits for the device and raises if a kernel reported a timeout.
@@ -60,7 +63,11 @@ private:
     LowerCSC A_;
     Factor factor_;
     cholespy_gpu_solver *gpu_ = nullptr;
-    int n_levels_lower_ = 0, n_levels_upper_ = 0;
+    // The level analysis of L and L' (src/cholesky_solver.cpp:299-376) is
+    // only needed by the row schedule and by levels(): it runs on first use.
+    void analyse() const;
+    mutable bool analysed_ = false;
+    mutable int n_levels_lower_ = 0, n_levels_upper_ = 0;
 };
 
 // One system of a batch, as given to the constructor (host arrays).

@@ -12,102 +12,12 @@
 #include <string>
 #include <thread>
 
+#include "dense.hpp"
 #include "factor.hpp"
 
 namespace cholespy_amd {
 
 namespace {
-
-// ---------------------------------------------------------------------------
-// Dense kernels (column-major, double). The product kernel is an 8 x 4
-// register tile of C updated with one rank-1 term per step (two 4-wide loads
-// of A, four broadcasts of B, eight FMAs), row panels of A blocked to stay in
-// cache. It is compiled twice: for AVX2+FMA (selected at run time) and for
-// the baseline instruction set.
-
-typedef double v4d __attribute__((vector_size(32)));
-
-constexpr int kMR = 8, kNR = 4, kMC = 128;
-
-// C(m x n) -= A(m x k) B(n x k)^T. Lower: C is a diagonal block (m == n) and
-// only tiles that reach the lower triangle are computed (tiles straddling the
-// diagonal also write above it; callers ignore the strict upper triangle).
-template <bool Lower>
-static inline __attribute__((always_inline)) void gemm_nt_body(int m, int n, int k, const double *A, int lda,
-                                                              const double *B, int ldb, double *C, int ldc) {
-    for (int ib = 0; ib < m; ib += kMC) {
-        const int ie = std::min(m, ib + kMC);
-        const int jend = Lower ? std::min(n, ie) : n;
-        int j0 = 0;
-        for (; j0 + kNR <= jend; j0 += kNR) {
-            int i0 = ib;
-            if (Lower) i0 = std::max(ib, (j0 / kMR) * kMR);
-            for (; i0 + kMR <= ie; i0 += kMR) {
-                v4d c0[kNR], c1[kNR];
-                for (int q = 0; q < kNR; ++q) c0[q] = c1[q] = v4d{0.0, 0.0, 0.0, 0.0};
-                const double *a = A + i0;
-                const double *b = B + j0;
-                for (int p = 0; p < k; ++p) {
-                    v4d a0, a1;
-                    std::memcpy(&a0, a + (size_t)p * lda, sizeof(v4d));
-                    std::memcpy(&a1, a + (size_t)p * lda + 4, sizeof(v4d));
-                    const double *bp = b + (size_t)p * ldb;
-                    for (int q = 0; q < kNR; ++q) {
-                        const double bv = bp[q];
-                        c0[q] += a0 * bv;
-                        c1[q] += a1 * bv;
-                    }
-                }
-                for (int q = 0; q < kNR; ++q) {
-                    double *cp = C + (size_t)(j0 + q) * ldc + i0;
-                    v4d x0, x1;
-                    std::memcpy(&x0, cp, sizeof(v4d));
-                    std::memcpy(&x1, cp + 4, sizeof(v4d));
-                    x0 -= c0[q];
-                    x1 -= c1[q];
-                    std::memcpy(cp, &x0, sizeof(v4d));
-                    std::memcpy(cp + 4, &x1, sizeof(v4d));
-                }
-            }
-            for (; i0 < ie; ++i0)
-                for (int q = 0; q < kNR; ++q) {
-                    double s = 0.0;
-                    for (int p = 0; p < k; ++p) s += A[i0 + (size_t)p * lda] * B[j0 + q + (size_t)p * ldb];
-                    C[i0 + (size_t)(j0 + q) * ldc] -= s;
-                }
-        }
-        for (; j0 < jend; ++j0)
-            for (int i = Lower ? std::max(ib, j0) : ib; i < ie; ++i) {
-                double s = 0.0;
-                for (int p = 0; p < k; ++p) s += A[i + (size_t)p * lda] * B[j0 + (size_t)p * ldb];
-                C[i + (size_t)j0 * ldc] -= s;
-            }
-    }
-}
-
-__attribute__((target("avx2,fma"))) void gemm_nt_avx2(bool lower, int m, int n, int k, const double *A, int lda,
-                                                      const double *B, int ldb, double *C, int ldc) {
-    if (lower)
-        gemm_nt_body<true>(m, n, k, A, lda, B, ldb, C, ldc);
-    else
-        gemm_nt_body<false>(m, n, k, A, lda, B, ldb, C, ldc);
-}
-void gemm_nt_base(bool lower, int m, int n, int k, const double *A, int lda, const double *B, int ldb, double *C,
-                  int ldc) {
-    if (lower)
-        gemm_nt_body<true>(m, n, k, A, lda, B, ldb, C, ldc);
-    else
-        gemm_nt_body<false>(m, n, k, A, lda, B, ldb, C, ldc);
-}
-
-using GemmFn = void (*)(bool, int, int, int, const double *, int, const double *, int, double *, int);
-GemmFn gemm_nt() {
-    static const GemmFn fn = [] {
-        __builtin_cpu_init();
-        return (__builtin_cpu_supports("avx2") && __builtin_cpu_supports("fma")) ? &gemm_nt_avx2 : &gemm_nt_base;
-    }();
-    return fn;
-}
 
 // Runs fn(part) for part in [0, parts) on up to `threads` threads.
 void parallel_parts(int parts, int threads, const std::function<void(int)> &fn) {
@@ -135,7 +45,7 @@ struct NotPositiveDefinite {};
 // update matrix F22 - L21 L21'. Blocked right-looking: unblocked factor of
 // a kNB diagonal block, TRSM of the rows below it, SYRK of the trailing part.
 void partial_factor(double *F, int N, int ns, int threads) {
-    const GemmFn gemm = gemm_nt();
+    const dense::GemmFn gemm = dense::gemm_nt();
     for (int j0 = 0; j0 < ns; j0 += kNB) {
         const int jb = std::min(kNB, ns - j0);
         // Diagonal block (already updated by earlier block columns).
@@ -179,7 +89,7 @@ void partial_factor(double *F, int N, int ns, int threads) {
             // area of columns [0, c) of an m x m lower triangle = c m - c^2 / 2
             const double target = (double)q / cparts * 0.5 * (double)m * (double)m;
             const double c = (double)m - std::sqrt(std::max(0.0, (double)m * m - 2.0 * target));
-            cut[q] = std::min(m, std::max(cut[q - 1], ((int)c + kNR - 1) / kNR * kNR));
+            cut[q] = std::min(m, std::max(cut[q - 1], ((int)c + dense::kNR - 1) / dense::kNR * dense::kNR));
         }
         cut[cparts] = m;
         parallel_parts(cparts, threads, [&](int part) {

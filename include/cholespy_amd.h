@@ -115,7 +115,10 @@ int cholespy_solver_get_levels(const cholespy_gpu_solver *solver, int lower, int
  * launch_kernel() (src/cholesky_solver.cpp:378-432) and the kernels
  * solve_{lower,upper}_{float,double} (kernels/kernels.cu:64-218). Returns
  * CHOLESPY_ERR_INVALID_ARGUMENT for n_rhs > 128, as the reference throws
- * "The number of RHS should be less than 128." (src/cholesky_solver.cpp:419). */
+ * "The number of RHS should be less than 128." (src/cholesky_solver.cpp:419).
+ * The default block schedule needs only upload_factor; the row schedule
+ * (cholespy_solver_set_path) also needs both passes analysed
+ * (CHOLESPY_ERR_STATE otherwise). */
 int cholespy_solver_solve_f32(cholespy_gpu_solver *solver, int n_rhs, const float *b,
                               float *x, void *stream);
 int cholespy_solver_solve_f64(cholespy_gpu_solver *solver, int n_rhs, const double *b,
