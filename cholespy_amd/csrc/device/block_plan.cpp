@@ -24,8 +24,7 @@ inline int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {      // row-major, c
 
 }  // namespace
 
-BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const double *val,
-                           const BlockPlanOptions &opt) {
+BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx, const BlockPlanOptions &opt) {
     BlockPlan P;
     P.n = n;
     P.nnz_l = colptr[n];
@@ -63,7 +62,7 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
     for (int b = 0; b < nb; ++b)
         for (int i = start[b]; i < start[b + 1]; ++i) block_of[i] = b;
 
-    // 2. Per block: R rows, dense L_BB and L_RB, inverse and products.
+    // 2. Per block: R rows and the offset of M_B.
     P.rptr.assign((size_t)nb + 1, 0);
     P.moff.assign((size_t)nb + 1, 0);
     std::vector<int> mark(n, -1);
@@ -87,65 +86,6 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
         P.max_m = std::max<int>(P.max_m, (int)m);
         P.max_rows = std::max<int>(P.max_rows, (int)(m + R));
     }
-    P.mfwd.assign((size_t)P.moff[nb], 0.0);
-    P.mbwd.assign((size_t)P.moff[nb], 0.0);
-    // Dense inverses per block: independent, so blocks are shared out over
-    // host threads (heaviest first, taken from a shared counter).
-    std::vector<int> by_cost(nb);
-    std::iota(by_cost.begin(), by_cost.end(), 0);
-    std::stable_sort(by_cost.begin(), by_cost.end(), [&](int x, int y) {
-        return P.moff[x + 1] - P.moff[x] > P.moff[y + 1] - P.moff[y];
-    });
-    std::atomic<int> next_block(0);
-    auto dense_work = [&]() {
-        std::vector<int> pos(n, -1);
-        std::vector<double> Lbb, Pb, Linv, Q;
-        for (int t = next_block.fetch_add(1); t < nb; t = next_block.fetch_add(1)) {
-            const int b = by_cost[t];
-            const int a = start[b], e = start[b + 1];
-            const int m = e - a;
-            const int R = P.rptr[b + 1] - P.rptr[b];
-            const int *rl = P.rlist.data() + P.rptr[b];
-            for (int q = 0; q < R; ++q) pos[rl[q]] = q;
-            // Column-major L_BB (m x m) and L_RB (R x m); X := I L_BB^{-T}
-            // = L_BB^{-T} (so L_BB^{-1}[rho][c] = X[c][rho]); then
-            // C = -L_RB X^T = -L_RB L_BB^{-1}, the lower part of M_B.
-            Lbb.assign((size_t)m * m, 0.0);
-            Pb.assign((size_t)R * m, 0.0);
-            for (int c = a; c < e; ++c)
-                for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
-                    const int i = rowidx[p];
-                    if (i < e) Lbb[(size_t)(c - a) * m + (i - a)] = val[p];
-                    else Pb[(size_t)(c - a) * R + pos[i]] = val[p];
-                }
-            Linv.assign((size_t)m * m, 0.0);
-            for (int c = 0; c < m; ++c) Linv[(size_t)c * m + c] = 1.0;
-            dense::trsm_right_lower_trans(m, m, Lbb.data(), m, Linv.data(), m);
-            Q.assign((size_t)R * m, 0.0);
-            if (R > 0) dense::gemm_nt()(false, R, m, m, Pb.data(), R, Linv.data(), m, Q.data(), R);
-            const int64_t rows = (int64_t)m + R;
-            double *F = P.mfwd.data() + P.moff[b];
-            double *Bk = P.mbwd.data() + P.moff[b];
-            for (int rho = 0; rho < rows; ++rho) {
-                const int cmax = rho < m ? rho : m - 1;
-                for (int c = 0; c <= cmax; ++c) {
-                    const double v = rho < m ? Linv[(size_t)rho * m + c] : Q[(size_t)c * R + (rho - m)];
-                    F[fwd_off(rows, c, rho)] = v;
-                    Bk[bwd_off(m, rho, c)] = v;
-                }
-            }
-            for (int q = 0; q < R; ++q) pos[rl[q]] = -1;
-        }
-    };
-    {
-        const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        const int nt = std::max(1, std::min(hw, opt.threads > 0 ? opt.threads : hw));
-        std::vector<std::thread> pool;
-        for (int t = 1; t < nt; ++t) pool.emplace_back(dense_work);
-        dense_work();
-        for (auto &t : pool) t.join();
-    }
-
     // 3. Contribution lists: slot s (row rlist[s]) feeds row rlist[s].
     const int nslots = P.rptr[nb];
     P.contrib_ptr.assign((size_t)n + 1, 0);
@@ -284,6 +224,76 @@ BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const do
         lo = hi;
     }
     P.nsplit_rows = std::max(fwd_split_rows, next_split);
+    return P;
+}
+
+void fill_block_plan_values(BlockPlan &P, const int *colptr, const int *rowidx, const double *val, int threads) {
+    const int nb = P.nblocks;
+    P.mfwd.assign((size_t)P.moff[nb], 0.0);
+    P.mbwd.assign((size_t)P.moff[nb], 0.0);
+    // Dense inverses per block: independent, so blocks are shared out over
+    // host threads (heaviest first, taken from a shared counter).
+    std::vector<int> by_cost(nb);
+    std::iota(by_cost.begin(), by_cost.end(), 0);
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](int x, int y) {
+        return P.moff[x + 1] - P.moff[x] > P.moff[y + 1] - P.moff[y];
+    });
+    std::atomic<int> next_block(0);
+    auto dense_work = [&]() {
+        std::vector<int> pos(P.n, -1);
+        std::vector<double> Lbb, Pb, Linv, Q;
+        for (int t = next_block.fetch_add(1); t < nb; t = next_block.fetch_add(1)) {
+            const int b = by_cost[t];
+            const int a = P.start[b], e = P.start[b + 1];
+            const int m = e - a;
+            const int R = P.rptr[b + 1] - P.rptr[b];
+            const int *rl = P.rlist.data() + P.rptr[b];
+            for (int q = 0; q < R; ++q) pos[rl[q]] = q;
+            // Column-major L_BB (m x m) and L_RB (R x m); X := I L_BB^{-T}
+            // = L_BB^{-T} (so L_BB^{-1}[rho][c] = X[c][rho]); then
+            // C = -L_RB X^T = -L_RB L_BB^{-1}, the lower part of M_B.
+            Lbb.assign((size_t)m * m, 0.0);
+            Pb.assign((size_t)R * m, 0.0);
+            for (int c = a; c < e; ++c)
+                for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
+                    const int i = rowidx[p];
+                    if (i < e) Lbb[(size_t)(c - a) * m + (i - a)] = val[p];
+                    else Pb[(size_t)(c - a) * R + pos[i]] = val[p];
+                }
+            Linv.assign((size_t)m * m, 0.0);
+            for (int c = 0; c < m; ++c) Linv[(size_t)c * m + c] = 1.0;
+            dense::trsm_right_lower_trans(m, m, Lbb.data(), m, Linv.data(), m);
+            Q.assign((size_t)R * m, 0.0);
+            if (R > 0) dense::gemm_nt()(false, R, m, m, Pb.data(), R, Linv.data(), m, Q.data(), R);
+            const int64_t rows = (int64_t)m + R;
+            double *F = P.mfwd.data() + P.moff[b];
+            double *Bk = P.mbwd.data() + P.moff[b];
+            for (int rho = 0; rho < rows; ++rho) {
+                const int cmax = rho < m ? rho : m - 1;
+                for (int c = 0; c <= cmax; ++c) {
+                    const double v = rho < m ? Linv[(size_t)rho * m + c] : Q[(size_t)c * R + (rho - m)];
+                    F[fwd_off(rows, c, rho)] = v;
+                    Bk[bwd_off(m, rho, c)] = v;
+                }
+            }
+            for (int q = 0; q < R; ++q) pos[rl[q]] = -1;
+        }
+    };
+    {
+        const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const int nt = std::max(1, std::min(hw, threads > 0 ? threads : hw));
+        std::vector<std::thread> pool;
+        for (int t = 1; t < nt; ++t) pool.emplace_back(dense_work);
+        dense_work();
+        for (auto &t : pool) t.join();
+    }
+
+}
+
+BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const double *val,
+                           const BlockPlanOptions &opt) {
+    BlockPlan P = build_block_plan_structure(n, colptr, rowidx, opt);
+    fill_block_plan_values(P, colptr, rowidx, val, opt.threads);
     return P;
 }
 

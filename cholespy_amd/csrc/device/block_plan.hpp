@@ -87,4 +87,12 @@ struct BlockPlan {
 BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const double *val,
                            const BlockPlanOptions &opt);
 
+// The two halves of build_block_plan. The structure (blocks, R rows, slots,
+// levels, tasks) depends only on the pattern of L; fill_block_plan_values
+// (re)computes mfwd / mbwd for the values val of L with that pattern, on up
+// to `threads` host threads (0 = hardware count, at most 16). A numeric
+// refactorisation keeps the structure and refills the values.
+BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx, const BlockPlanOptions &opt);
+void fill_block_plan_values(BlockPlan &P, const int *colptr, const int *rowidx, const double *val, int threads);
+
 }  // namespace cholespy_amd

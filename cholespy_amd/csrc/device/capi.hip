@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -154,6 +155,11 @@ struct cholespy_gpu_solver {
     long long *sys_vbase = nullptr, *sys_pbase = nullptr, *sys_sbase = nullptr;
     std::vector<int> last_k;   // RHS counts the per-system arrays hold
     std::vector<int> host_levels[2], host_row_level[2];
+    // Pattern of the uploaded factor, kept for cholespy_solver_refactor:
+    // column form of L, for each position of the row form the column-form
+    // position it copies, and the block plan's structure (values cleared).
+    std::vector<int> host_colptr, host_rowidx, csr_src;
+    BlockPlan host_plan;
 };
 
 namespace {
@@ -225,6 +231,50 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
     for (auto &row : s->grid_blocks)
         for (int &g : row) g = 0;
     s->buffers_path = -1;
+    return CHOLESPY_OK;
+}
+
+// Copies the values of L (column form, in the order of host_rowidx) into the
+// row and column forms on the device, in the handle's precision.
+int upload_factor_values(cholespy_gpu_solver *s, const double *val) {
+    const int64_t nnz = s->nnz;
+    const size_t fb = (size_t)s->fbytes;
+    std::vector<char> lv((size_t)nnz * fb), uv((size_t)nnz * fb);
+    if (s->fbytes == 4) {
+        float *l = reinterpret_cast<float *>(lv.data()), *u = reinterpret_cast<float *>(uv.data());
+        for (int64_t p = 0; p < nnz; ++p) {
+            l[p] = (float)val[s->csr_src[(size_t)p]];
+            u[p] = (float)val[p];
+        }
+    } else {
+        double *l = reinterpret_cast<double *>(lv.data());
+        for (int64_t p = 0; p < nnz; ++p) l[p] = val[s->csr_src[(size_t)p]];
+        std::memcpy(uv.data(), val, (size_t)nnz * 8);
+    }
+    HIP_TRY(hipMemcpy(s->lvals, lv.data(), (size_t)nnz * fb, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s->uvals, uv.data(), (size_t)nnz * fb, hipMemcpyHostToDevice));
+    return CHOLESPY_OK;
+}
+
+// Copies the values of M_B (both layouts) into the device plan, whose
+// structure they must match.
+int upload_plan_values(cholespy_gpu_solver *s, const BlockPlan &P) {
+    DeviceBlockPlan &d = s->plan;
+    const size_t me = (size_t)d.m_entries;
+    if (P.mfwd.size() != me || P.mbwd.size() != me)
+        return fail(CHOLESPY_ERR_STATE, "block plan values do not match the uploaded plan");
+    if (s->fbytes == 4) {
+        std::vector<float> f(me), b(me);
+        for (size_t i = 0; i < me; ++i) {
+            f[i] = (float)P.mfwd[i];
+            b[i] = (float)P.mbwd[i];
+        }
+        HIP_TRY(hipMemcpy(d.mfwd, f.data(), me * sizeof(float), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.mbwd, b.data(), me * sizeof(float), hipMemcpyHostToDevice));
+    } else {
+        HIP_TRY(hipMemcpy(d.mfwd, P.mfwd.data(), me * sizeof(double), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.mbwd, P.mbwd.data(), me * sizeof(double), hipMemcpyHostToDevice));
+    }
     return CHOLESPY_OK;
 }
 
@@ -362,7 +412,7 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     }
     // Row-compressed L (transpose of the column form): diagonal ends each row.
     std::vector<int> rowptr((size_t)n + 1, 0), cols((size_t)nnz);
-    std::vector<double> rvals((size_t)nnz);
+    std::vector<int> csr_src((size_t)nnz);
     for (int64_t p = 0; p < nnz; ++p) rowptr[(size_t)rowidx[p] + 1]++;
     for (int i = 0; i < n; ++i) rowptr[i + 1] += rowptr[i];
     {
@@ -371,21 +421,11 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
             for (int p = colptr[j]; p < colptr[j + 1]; ++p) {
                 const int dst = next[rowidx[p]]++;
                 cols[dst] = j;
-                rvals[dst] = val[p];
+                csr_src[dst] = p;
             }
     }
     const size_t fb = (size_t)s->fbytes;
-    std::vector<char> lv((size_t)nnz * fb), uv((size_t)nnz * fb);
-    if (s->fbytes == 4) {
-        float *l = reinterpret_cast<float *>(lv.data()), *u = reinterpret_cast<float *>(uv.data());
-        for (int64_t p = 0; p < nnz; ++p) {
-            l[p] = (float)rvals[p];
-            u[p] = (float)val[p];
-        }
-    } else {
-        std::memcpy(lv.data(), rvals.data(), (size_t)nnz * 8);
-        std::memcpy(uv.data(), val, (size_t)nnz * 8);
-    }
+    s->uploaded = false;
     dev_free(s->lrowptr);
     dev_free(s->lcols);
     dev_free(s->lvals);
@@ -401,20 +441,49 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     HIP_TRY(hipMemcpy(s->perm, perm, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->lrowptr, rowptr.data(), ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->lcols, cols.data(), (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->lvals, lv.data(), (size_t)nnz * fb, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->urowptr, colptr, ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(s->ucols, rowidx, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(s->uvals, uv.data(), (size_t)nnz * fb, hipMemcpyHostToDevice));
     s->nnz = nnz;
-    s->uploaded = true;
+    s->csr_src.swap(csr_src);
+    s->host_colptr.assign(colptr, colptr + n + 1);
+    s->host_rowidx.assign(rowidx, rowidx + nnz);
+    int rc = upload_factor_values(s, val);
+    if (rc != CHOLESPY_OK) return rc;
     s->analysed[0] = s->analysed[1] = false;
 
     BlockPlanOptions opt = plan_options();
     BlockPlan P = build_block_plan(n, colptr, rowidx, val, opt);
-    int rc = upload_plan(s, P);
+    rc = upload_plan(s, P);
     if (rc != CHOLESPY_OK) return rc;
+    // The structure stays on the host for refactorisations; the values are on the device.
+    std::vector<double>().swap(P.mfwd);
+    std::vector<double>().swap(P.mbwd);
+    s->host_plan = std::move(P);
+    s->uploaded = true;
     s->buffers_path = -1;
     return CHOLESPY_OK;
+}
+
+int cholespy_solver_refactor(cholespy_gpu_solver *s, const double *val, int64_t nnz) {
+    if (!s || !val) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "refactor is not available for batches");
+    if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "refactor called before upload_factor");
+    if (nnz != s->nnz) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "nnz does not match the uploaded factor");
+    for (int j = 0; j < s->n; ++j) {
+        const double d = val[s->host_colptr[j]];
+        if (!(d > 0.0) || !std::isfinite(d))
+            return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "factor diagonal must be positive and finite");
+    }
+    BlockPlan &P = s->host_plan;
+    fill_block_plan_values(P, s->host_colptr.data(), s->host_rowidx.data(), val, plan_options().threads);
+    // Solves still in flight read the old values: wait for them before
+    // overwriting anything.
+    HIP_TRY(hipDeviceSynchronize());
+    int rc = upload_factor_values(s, val);
+    if (rc == CHOLESPY_OK) rc = upload_plan_values(s, P);
+    std::vector<double>().swap(P.mfwd);
+    std::vector<double>().swap(P.mbwd);
+    return rc;
 }
 
 int cholespy_solver_analyze(cholespy_gpu_solver *s, int lower, int *n_levels) {

@@ -279,6 +279,21 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
                 }
             },
             py::arg("b"), py::arg("x"), doc_solve)
+        .def(
+            "refactorize",
+            [](Class &self, py::object x_o) {
+                x_o = as_exportable(x_o);
+                DLView x(x_o);
+                if (x.ndim() != 1) throw py::type_error("x must be a one-dimensional array.");
+                if (x.device() != Dev::CPU && x.device() != Dev::GPU)
+                    throw std::invalid_argument("Unsupported input device! Only CPU and CUDA arrays are supported.");
+                if (x.shape(0) != self.input_nnz())
+                    throw std::invalid_argument("refactorize(): x must have as many entries as at construction.");
+                std::vector<double> xv = to_host<double>(x);
+                py::gil_scoped_release release;
+                self.refactorize((int64_t)xv.size(), xv.data());
+            },
+            py::arg("x"), doc_refactorize)
         // Inspection hooks (not part of the reference API): the host factor,
         // the GPU level analysis and the device watchdog status.
         .def("_factor",

@@ -20,8 +20,9 @@ template <typename Float>
 CholeskySolver<Float>::CholeskySolver(int n_rows, int64_t nnz, const int *ii, const int *jj,
                                       const double *x, MatrixType type, bool cpu)
     : n_(n_rows), cpu_(cpu) {
-    A_ = normalize_input(n_rows, nnz, ii, jj, x, type);
-    factor_ = cholesky_factor(A_);
+    A_ = normalize_input(n_rows, nnz, ii, jj, x, type, &entry_dst_);
+    symbolic_ = symbolic_factor(A_);
+    factor_ = numeric_factor(symbolic_, A_.val);
     if (cpu_) return;
 
     // GPU tail of the reference's factorize() (src/cholesky_solver.cpp:244-297):
@@ -76,6 +77,22 @@ void CholeskySolver<Float>::solve_cpu(int n_rhs, const Float *b, Float *x) {
     factor_solve_colmajor(factor_, n_rhs, buf.data());
     for (int i = 0; i < n_; ++i)
         for (int j = 0; j < n_rhs; ++j) x[(size_t)i * n_rhs + j] = (Float)buf[(size_t)j * n_ + i];
+}
+
+template <typename Float>
+void CholeskySolver<Float>::refactorize(int64_t nnz, const double *x) {
+    if (nnz != input_nnz())
+        throw std::invalid_argument("refactorize(): x must have as many entries as at construction.");
+    std::vector<double> aval(A_.val.size(), 0.0);
+    refill_values(entry_dst_, x, aval);
+    Factor f = numeric_factor(symbolic_, aval);
+    if (gpu_) {
+        throw_on_error(cholespy_solver_refactor(gpu_, f.val.data(), f.nnz()));
+        throw_on_error(cholespy_solver_upload_matrix(gpu_, A_.colptr.data(), A_.rowidx.data(), aval.data(),
+                                                     A_.nnz()));
+    }
+    A_.val.swap(aval);
+    factor_.val.swap(f.val);
 }
 
 template <typename Float>

@@ -31,6 +31,17 @@ public:
     // src/cholesky_solver.cpp:434-463: row-major Float in/out, FP64 inside.
     void solve_cpu(int n_rhs, const Float *b, Float *x);
 
+    // Numeric refactorisation for new values x of the constructor's pattern:
+    // nnz values in the order of the constructor's x (same ii / jj, same
+    // matrix type). Reuses the ordering, the symbolic factor, the level
+    // analysis and the block plan's structure; the result equals a new
+    // solver built from the same ii / jj and x. No counterpart in the
+    // reference, which builds a new solver for every new matrix
+    // (tests/Benchmark.ipynb:238-254). A matrix that is not positive definite
+    // raises and leaves the solver unchanged.
+    void refactorize(int64_t nnz, const double *x);
+    int64_t input_nnz() const { return (int64_t)entry_dst_.size(); }
+
     bool is_cpu() const { return cpu_; }
     int n() const { return n_; }
     const Factor &factor() const { return factor_; }
@@ -61,6 +72,8 @@ private:
     int n_ = 0;
     bool cpu_ = true;
     LowerCSC A_;
+    std::vector<int64_t> entry_dst_;   // input entry -> position in A_.val (-1: dropped)
+    SymbolicFactor symbolic_;
     Factor factor_;
     cholespy_gpu_solver *gpu_ = nullptr;
     // The level analysis of L and L' (src/cholesky_solver.cpp:299-376) is

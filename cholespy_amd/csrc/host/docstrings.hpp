@@ -45,3 +45,22 @@ Both arrays must already have the solver's dtype (float32 for
 CholeskySolverF, float64 for CholeskySolverD) and be C-contiguous: `x` is
 written in place, so no conversion is made.
 )";
+
+static const char *doc_refactorize = R"(
+
+Refactorize for new values of the matrix with the same pattern.
+
+Parameters
+----------
+
+- `x` - New nonzero values: as many as the constructor's `x`, in the same
+        order (the constructor's `ii`, `jj` and matrix type are kept). CPU or
+        GPU array, converted to float64.
+
+The ordering, the symbolic factor, the GPU level analysis and the block
+plan's structure are reused; only the numeric factor and the dense block
+inverses are recomputed. The result equals a new solver built from the same
+`ii`, `jj` and this `x`. Waits for pending solves of this device. Raises
+ValueError("Matrix is not positive definite!") and leaves the solver
+unchanged when the new matrix is not positive definite.
+)";

@@ -21,7 +21,8 @@ struct UpperCSC {
     std::vector<double> val;
 };
 
-UpperCSC permute_upper(const LowerCSC &A, const std::vector<int> &pinv) {
+// a_to_c (optional) receives, per entry of A, its position in the result.
+UpperCSC permute_upper(const LowerCSC &A, const std::vector<int> &pinv, std::vector<int> *a_to_c = nullptr) {
     const int n = A.n;
     UpperCSC C;
     C.n = n;
@@ -34,6 +35,7 @@ UpperCSC permute_upper(const LowerCSC &A, const std::vector<int> &pinv) {
     for (int k = 0; k < n; ++k) C.colptr[k + 1] += C.colptr[k];
     C.rowidx.resize((size_t)C.colptr[n]);
     C.val.resize((size_t)C.colptr[n]);
+    if (a_to_c) a_to_c->assign((size_t)A.nnz(), -1);
     std::vector<int> next(C.colptr.begin(), C.colptr.end() - 1);
     for (int j = 0; j < n; ++j)
         for (int p = A.colptr[j]; p < A.colptr[j + 1]; ++p) {
@@ -42,6 +44,7 @@ UpperCSC permute_upper(const LowerCSC &A, const std::vector<int> &pinv) {
             const int dst = next[col]++;
             C.rowidx[dst] = row;
             C.val[dst] = A.val[p];
+            if (a_to_c) (*a_to_c)[(size_t)p] = dst;
         }
     return C;
 }
@@ -121,10 +124,11 @@ inline int row_pattern(const UpperCSC &C, int k, const std::vector<int> &parent,
 
 }  // namespace
 
-Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
+SymbolicFactor symbolic_factor(const LowerCSC &A, const FactorOptions &opt) {
     const int n = A.n;
-    Factor f;
-    f.n = n;
+    SymbolicFactor S;
+    S.n = n;
+    S.threads = opt.threads;
 
     // 1. Fill-reducing ordering on the adjacency graph of A.
     std::vector<int> perm0(n);
@@ -164,33 +168,31 @@ Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
         UpperCSC C0 = permute_upper(A, pinv);
         std::vector<int> par0 = etree_upper(C0);
         std::vector<int> post = postorder(par0);
-        f.perm.resize(n);
-        for (int k = 0; k < n; ++k) f.perm[k] = perm0[post[k]];
+        S.perm.resize(n);
+        for (int k = 0; k < n; ++k) S.perm[k] = perm0[post[k]];
     }
-    for (int k = 0; k < n; ++k) pinv[f.perm[k]] = k;
-    UpperCSC C = permute_upper(A, pinv);
-    f.parent = etree_upper(C);
+    for (int k = 0; k < n; ++k) pinv[S.perm[k]] = k;
+    UpperCSC C = permute_upper(A, pinv, &S.a_to_c);
+    S.parent = etree_upper(C);
 
     // 3. Column counts from the row patterns (row subtrees).
     std::vector<int> mark(n, -1), path(n), buf(n);
-    std::vector<int64_t> count(n, 1);
+    S.count.assign(n, 1);
     for (int k = 0; k < n; ++k) {
-        const int top = row_pattern(C, k, f.parent, mark, path, buf);
-        for (int t = top; t < n; ++t) count[buf[t]]++;
+        const int top = row_pattern(C, k, S.parent, mark, path, buf);
+        for (int t = top; t < n; ++t) S.count[buf[t]]++;
     }
-    f.colptr.assign((size_t)n + 1, 0);
+    S.colptr.assign((size_t)n + 1, 0);
     int64_t total = 0;
     for (int j = 0; j < n; ++j) {
         if (total > std::numeric_limits<int>::max())
             throw std::runtime_error("Cholesky factor too large for 32-bit indices.");
-        f.colptr[j] = (int)total;
-        total += count[j];
+        S.colptr[j] = (int)total;
+        total += S.count[j];
     }
     if (total > std::numeric_limits<int>::max())
         throw std::runtime_error("Cholesky factor too large for 32-bit indices.");
-    f.colptr[n] = (int)total;
-    f.rowidx.resize((size_t)total);
-    f.val.resize((size_t)total);
+    S.colptr[n] = (int)total;
 
     FactorOptions::Method method = opt.method;
     if (method == FactorOptions::Method::Auto) {
@@ -203,28 +205,59 @@ Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
             method = total >= opt.supernodal_min_nnz ? FactorOptions::Method::Supernodal
                                                      : FactorOptions::Method::Simplicial;
     }
-    if (method == FactorOptions::Method::Supernodal) {
-        // 4'. Multifrontal numeric factorisation on the lower CSC of P A P'
-        //     (the transpose of C).
-        std::vector<int> lp((size_t)n + 1, 0), li((size_t)C.colptr[n]);
-        std::vector<double> lv((size_t)C.colptr[n]);
+    S.supernodal = method == FactorOptions::Method::Supernodal;
+    if (S.supernodal) {
+        // The lower CSC of P A P' (the transpose of C) for the multifrontal phase.
+        S.l_colptr.assign((size_t)n + 1, 0);
+        S.l_rowidx.resize((size_t)C.colptr[n]);
+        S.c_to_l.resize((size_t)C.colptr[n]);
         for (int k = 0; k < n; ++k)
-            for (int p = C.colptr[k]; p < C.colptr[k + 1]; ++p) lp[(size_t)C.rowidx[p] + 1]++;
-        for (int j = 0; j < n; ++j) lp[j + 1] += lp[j];
-        std::vector<int> next(lp.begin(), lp.end() - 1);
+            for (int p = C.colptr[k]; p < C.colptr[k + 1]; ++p) S.l_colptr[(size_t)C.rowidx[p] + 1]++;
+        for (int j = 0; j < n; ++j) S.l_colptr[j + 1] += S.l_colptr[j];
+        std::vector<int> next(S.l_colptr.begin(), S.l_colptr.end() - 1);
         for (int k = 0; k < n; ++k)
             for (int p = C.colptr[k]; p < C.colptr[k + 1]; ++p) {
                 const int q = next[C.rowidx[p]]++;
-                li[q] = k;
-                lv[q] = C.val[p];
+                S.l_rowidx[q] = k;
+                S.c_to_l[p] = q;
             }
-        supernodal_numeric(n, lp, li, lv, f.parent, count, f, opt.threads);
+    }
+    S.c_colptr = std::move(C.colptr);
+    S.c_rowidx = std::move(C.rowidx);
+    return S;
+}
+
+Factor numeric_factor(const SymbolicFactor &S, const std::vector<double> &a_val) {
+    const int n = S.n;
+    if (a_val.size() != S.a_to_c.size())
+        throw std::invalid_argument("numeric_factor: values do not match the symbolic factor.");
+    Factor f;
+    f.n = n;
+    f.perm = S.perm;
+    f.parent = S.parent;
+    f.colptr = S.colptr;
+    const int64_t total = S.colptr[n];
+    f.rowidx.resize((size_t)total);
+    f.val.resize((size_t)total);
+    const size_t nc = (size_t)S.c_colptr[n];
+
+    if (S.supernodal) {
+        // Multifrontal numeric factorisation on the lower CSC of P A P'.
+        std::vector<double> lv(nc);
+        for (size_t p = 0; p < a_val.size(); ++p) lv[(size_t)S.c_to_l[(size_t)S.a_to_c[p]]] = a_val[p];
+        supernodal_numeric(n, S.l_colptr, S.l_rowidx, lv, S.parent, S.count, f, S.threads);
         return f;
     }
 
-    // 4. Up-looking numeric factorisation: row k of L solves a sparse
-    //    triangular system against the already computed columns.
-    std::fill(mark.begin(), mark.end(), -1);
+    // Up-looking numeric factorisation: row k of L solves a sparse
+    // triangular system against the already computed columns.
+    UpperCSC C;
+    C.n = n;
+    C.colptr = S.c_colptr;
+    C.rowidx = S.c_rowidx;
+    C.val.resize(nc);
+    for (size_t p = 0; p < a_val.size(); ++p) C.val[(size_t)S.a_to_c[p]] = a_val[p];
+    std::vector<int> mark(n, -1), path(n), buf(n);
     std::vector<int> fill(f.colptr.begin(), f.colptr.end() - 1);   // next free slot per column
     std::vector<double> x(n, 0.0);
     for (int k = 0; k < n; ++k) {
@@ -249,6 +282,10 @@ Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
         f.val[q] = std::sqrt(d);
     }
     return f;
+}
+
+Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt) {
+    return numeric_factor(symbolic_factor(A, opt), A.val);
 }
 
 void factor_rows(const Factor &f, std::vector<int> &rowptr, std::vector<int> &colidx,

@@ -40,8 +40,35 @@ struct FactorOptions {
     int threads = 0;   // supernodal worker threads (0 = hardware count, at most 16)
 };
 
-// Throws std::invalid_argument("Matrix is not positive definite!") when a
-// pivot is not strictly positive (src/cholesky_solver.cpp:241-242).
+// Everything of the factorisation that depends only on the pattern of A
+// (CHOLMOD's symbolic factor, cholmod_analyze at src/cholesky_solver.cpp:226):
+// the ordering, the elimination tree, the column counts of L and the maps
+// that place A's values in the permuted matrix. A numeric factorisation
+// (numeric_factor) of any matrix with A's pattern reuses it.
+struct SymbolicFactor {
+    int n = 0;
+    std::vector<int> perm, parent;      // as in Factor
+    std::vector<int> colptr;            // column pointers of L
+    std::vector<int64_t> count;         // column counts of L (diagonal included)
+    // Upper CSC C of P A P' (column k holds the rows i <= k) and, for each
+    // entry of A's lower CSC, its position in C.
+    std::vector<int> c_colptr, c_rowidx, a_to_c;
+    // Supernodal numeric phase: the lower CSC of P A P' (rows ascending) and,
+    // for each entry of C, its position there.
+    std::vector<int> l_colptr, l_rowidx, c_to_l;
+    bool supernodal = false;
+    int threads = 0;
+};
+
+SymbolicFactor symbolic_factor(const LowerCSC &A, const FactorOptions &opt = FactorOptions());
+
+// L of P A P' for the values a_val of a matrix with the symbolic factor's
+// pattern (a_val in the order of A's lower CSC). Throws
+// std::invalid_argument("Matrix is not positive definite!") when a pivot is
+// not strictly positive (src/cholesky_solver.cpp:241-242).
+Factor numeric_factor(const SymbolicFactor &S, const std::vector<double> &a_val);
+
+// symbolic_factor followed by numeric_factor.
 Factor cholesky_factor(const LowerCSC &A, const FactorOptions &opt = FactorOptions());
 
 // Row-compressed copy of L: rows ascending, columns ascending, diagonal last.

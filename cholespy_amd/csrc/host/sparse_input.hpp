@@ -31,7 +31,17 @@ struct LowerCSC {
 //   CSR: ii = row pointer (n + 1), jj = column indices (nnz)
 //   CSC: ii = column pointer (n + 1), jj = row indices (nnz)
 // Throws std::invalid_argument on indices outside [0, n) or malformed pointers.
+// When entry_dst is given it receives, for every input entry (position in x),
+// the position in the result's val that the entry is summed into, or -1 for
+// entries above the diagonal: the map a numeric refactorisation uses to
+// rebuild val from new values in the same order (refill_values).
 LowerCSC normalize_input(int n, int64_t nnz, const int *ii, const int *jj,
-                         const double *x, MatrixType type);
+                         const double *x, MatrixType type,
+                         std::vector<int64_t> *entry_dst = nullptr);
+
+// val[q] := sum of x[e] over the entries e with entry_dst[e] == q, added in
+// input order starting from the first such entry: the same sums, bit for
+// bit, that normalize_input forms from these values.
+void refill_values(const std::vector<int64_t> &entry_dst, const double *x, std::vector<double> &val);
 
 }  // namespace cholespy_amd

@@ -96,6 +96,18 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *solver, const int *perm,
                                   const int *colptr, const int *rowidx,
                                   const double *val, int64_t nnz);
 
+/* Numeric refactorisation: replaces the values of the uploaded factor by
+ * val (nnz doubles, same pattern and order as the colptr / rowidx given to
+ * cholespy_solver_upload_factor; perm is kept). Keeps the row form's
+ * pattern, the level analysis and the block plan's structure, recomputing
+ * only the dense block inverses. Waits for all work on the device before
+ * overwriting values. Single-system handles only. The reference has no
+ * counterpart: its benchmark builds a new CholeskySolver for every new
+ * matrix with the same pattern (tests/Benchmark.ipynb:238-254), redoing
+ * the ordering, the symbolic factor and analyze_cuda
+ * (src/cholesky_solver.cpp:151-198, 299-376). */
+int cholespy_solver_refactor(cholespy_gpu_solver *solver, const double *val, int64_t nnz);
+
 /* Level-set analysis of L (lower = 1) or L' (lower = 0) on the GPU followed
  * by the stable counting sort of rows by level. Replaces analyze_cuda()
  * (src/cholesky_solver.cpp:299-376) and the kernels analysis_lower /
