@@ -231,63 +231,88 @@ void fill_block_plan_values(BlockPlan &P, const int *colptr, const int *rowidx, 
     const int nb = P.nblocks;
     P.mfwd.assign((size_t)P.moff[nb], 0.0);
     P.mbwd.assign((size_t)P.moff[nb], 0.0);
-    // Dense inverses per block: independent, so blocks are shared out over
-    // host threads (heaviest first, taken from a shared counter).
+    const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const int nt = std::max(1, std::min(hw, threads > 0 ? threads : hw));
+    // Dense inverses per block. Blocks are independent: the few large ones
+    // (top-of-tree separator chains, which would otherwise bound the time)
+    // run one after another with all threads inside their products, the
+    // rest are shared out over the threads, heaviest first.
+    struct Work {
+        std::vector<int> pos;
+        std::vector<double> Lbb, Pb, Linv, Q;
+    };
+    auto compute = [&](int b, int inner, Work &w) {
+        const int a = P.start[b], e = P.start[b + 1];
+        const int m = e - a;
+        const int R = P.rptr[b + 1] - P.rptr[b];
+        const int *rl = P.rlist.data() + P.rptr[b];
+        if (w.pos.empty()) w.pos.assign((size_t)P.n, -1);
+        for (int q = 0; q < R; ++q) w.pos[rl[q]] = q;
+        // Column-major L_BB (m x m) and L_RB (R x m); X := L_BB^{-T} (so
+        // L_BB^{-1}[rho][c] = X[c][rho]); then C = -L_RB L_BB^{-1}, the
+        // lower part of M_B.
+        w.Lbb.assign((size_t)m * m, 0.0);
+        w.Pb.assign((size_t)R * m, 0.0);
+        for (int c = a; c < e; ++c)
+            for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
+                const int i = rowidx[p];
+                if (i < e) w.Lbb[(size_t)(c - a) * m + (i - a)] = val[p];
+                else w.Pb[(size_t)(c - a) * R + w.pos[i]] = val[p];
+            }
+        for (int q = 0; q < R; ++q) w.pos[rl[q]] = -1;
+        w.Linv.assign((size_t)m * m, 0.0);
+        for (int c = 0; c < m; ++c) w.Linv[(size_t)c * m + c] = 1.0;
+        dense::lower_inverse_trans(m, w.Lbb.data(), m, w.Linv.data(), m, inner);
+        w.Q.assign((size_t)R * m, 0.0);
+        if (R > 0) dense::gemm_lower_inverse(R, m, w.Pb.data(), R, w.Linv.data(), m, w.Q.data(), R, inner);
+        const int rows = m + R;
+        double *F = P.mfwd.data() + P.moff[b];
+        double *Bk = P.mbwd.data() + P.moff[b];
+        const double *Linv = w.Linv.data(), *Q = w.Q.data();
+        // Both layouts written in kTile x kTile tiles, so that the
+        // column-major one is not written with a stride per entry.
+        constexpr int kTile = 32;
+        const int tiles = (rows + kTile - 1) / kTile;
+        dense::parallel_parts(inner, inner, [&](int part) {
+            const int t0 = dense::part_bound(tiles, inner, part, 1), t1 = dense::part_bound(tiles, inner, part + 1, 1);
+            for (int r0 = t0 * kTile; r0 < std::min(rows, t1 * kTile); r0 += kTile) {
+                const int r1 = std::min(rows, r0 + kTile);
+                for (int c0 = 0; c0 <= std::min(r1 - 1, m - 1); c0 += kTile)
+                    for (int rho = r0; rho < r1; ++rho) {
+                        const int cmax = std::min(rho < m ? rho : m - 1, c0 + kTile - 1);
+                        for (int c = c0; c <= cmax; ++c) {
+                            const double v = rho < m ? Linv[(size_t)rho * m + c] : Q[(size_t)c * R + (rho - m)];
+                            F[fwd_off(rows, c, rho)] = v;
+                            Bk[bwd_off(m, rho, c)] = v;
+                        }
+                    }
+            }
+        });
+    };
     std::vector<int> by_cost(nb);
     std::iota(by_cost.begin(), by_cost.end(), 0);
-    std::stable_sort(by_cost.begin(), by_cost.end(), [&](int x, int y) {
-        return P.moff[x + 1] - P.moff[x] > P.moff[y + 1] - P.moff[y];
-    });
-    std::atomic<int> next_block(0);
-    auto dense_work = [&]() {
-        std::vector<int> pos(P.n, -1);
-        std::vector<double> Lbb, Pb, Linv, Q;
-        for (int t = next_block.fetch_add(1); t < nb; t = next_block.fetch_add(1)) {
-            const int b = by_cost[t];
-            const int a = P.start[b], e = P.start[b + 1];
-            const int m = e - a;
-            const int R = P.rptr[b + 1] - P.rptr[b];
-            const int *rl = P.rlist.data() + P.rptr[b];
-            for (int q = 0; q < R; ++q) pos[rl[q]] = q;
-            // Column-major L_BB (m x m) and L_RB (R x m); X := I L_BB^{-T}
-            // = L_BB^{-T} (so L_BB^{-1}[rho][c] = X[c][rho]); then
-            // C = -L_RB X^T = -L_RB L_BB^{-1}, the lower part of M_B.
-            Lbb.assign((size_t)m * m, 0.0);
-            Pb.assign((size_t)R * m, 0.0);
-            for (int c = a; c < e; ++c)
-                for (int p = colptr[c]; p < colptr[c + 1]; ++p) {
-                    const int i = rowidx[p];
-                    if (i < e) Lbb[(size_t)(c - a) * m + (i - a)] = val[p];
-                    else Pb[(size_t)(c - a) * R + pos[i]] = val[p];
-                }
-            Linv.assign((size_t)m * m, 0.0);
-            for (int c = 0; c < m; ++c) Linv[(size_t)c * m + c] = 1.0;
-            dense::trsm_right_lower_trans(m, m, Lbb.data(), m, Linv.data(), m);
-            Q.assign((size_t)R * m, 0.0);
-            if (R > 0) dense::gemm_nt()(false, R, m, m, Pb.data(), R, Linv.data(), m, Q.data(), R);
-            const int64_t rows = (int64_t)m + R;
-            double *F = P.mfwd.data() + P.moff[b];
-            double *Bk = P.mbwd.data() + P.moff[b];
-            for (int rho = 0; rho < rows; ++rho) {
-                const int cmax = rho < m ? rho : m - 1;
-                for (int c = 0; c <= cmax; ++c) {
-                    const double v = rho < m ? Linv[(size_t)rho * m + c] : Q[(size_t)c * R + (rho - m)];
-                    F[fwd_off(rows, c, rho)] = v;
-                    Bk[bwd_off(m, rho, c)] = v;
-                }
-            }
-            for (int q = 0; q < R; ++q) pos[rl[q]] = -1;
-        }
+    auto cost = [&](int b) {
+        const double m = P.start[b + 1] - P.start[b], R = P.rptr[b + 1] - P.rptr[b];
+        return m * m * m / 3.0 + R * m * m;
     };
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    constexpr double kBigBlock = 4.0e6;   // multiply-adds worth splitting over threads
+    int nbig = 0;
+    if (nt > 1)
+        while (nbig < nb && cost(by_cost[nbig]) >= kBigBlock) ++nbig;
     {
-        const int hw = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        const int nt = std::max(1, std::min(hw, threads > 0 ? threads : hw));
-        std::vector<std::thread> pool;
-        for (int t = 1; t < nt; ++t) pool.emplace_back(dense_work);
-        dense_work();
-        for (auto &t : pool) t.join();
+        Work w;
+        for (int t = 0; t < nbig; ++t) compute(by_cost[t], nt, w);
     }
-
+    std::atomic<int> next_block(nbig);
+    auto dense_work = [&]() {
+        Work w;
+        for (int t = next_block.fetch_add(1); t < nb; t = next_block.fetch_add(1)) compute(by_cost[t], 1, w);
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(dense_work);
+    dense_work();
+    for (auto &t : pool) t.join();
 }
 
 BlockPlan build_block_plan(int n, const int *colptr, const int *rowidx, const double *val,

@@ -7,9 +7,36 @@
 #pragma once
 #include <algorithm>
 #include <cstring>
+#include <functional>
+#include <thread>
+#include <vector>
 
 namespace cholespy_amd {
 namespace dense {
+
+// Runs fn(part) for part in [0, parts) on up to `threads` threads.
+inline void parallel_parts(int parts, int threads, const std::function<void(int)> &fn) {
+    if (threads <= 1 || parts <= 1) {
+        for (int p = 0; p < parts; ++p) fn(p);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const int t = std::min(threads, parts);
+    for (int w = 1; w < t; ++w)
+        pool.emplace_back([&, w] {
+            for (int p = w; p < parts; p += t) fn(p);
+        });
+    for (int p = 0; p < parts; p += t) fn(p);
+    for (auto &th : pool) th.join();
+}
+
+// Bounds of part `part` of `parts` equal ranges of [0, total), rounded to
+// multiples of `align`.
+inline int part_bound(int total, int parts, int part, int align) {
+    if (part >= parts) return total;
+    const int b = (int)((int64_t)total * part / parts);
+    return std::min(total, (b + align - 1) / align * align);
+}
 
 typedef double v4d __attribute__((vector_size(32)));
 
@@ -116,6 +143,61 @@ inline void trsm_right_lower_trans(int m, int n, const double *L, int ldl, doubl
         if (j1 < n) gemm(false, m, n - j1, j1 - j0, X + (size_t)j0 * ldx, ldx, L + j1 + (size_t)j0 * ldl, ldl,
                          X + (size_t)j1 * ldx, ldx);
     }
+}
+
+// X := L^{-T} for L (m x m) lower with a nonzero diagonal, X holding the
+// identity on entry. Column j of the result is zero below row j, so the
+// column operations and the block updates touch rows 0..j only: a third of
+// the work of trsm_right_lower_trans on the identity, with the same
+// operations on every entry that is not structurally zero.
+// threads > 1 splits each block update by columns (the same operations per
+// entry, so the result does not depend on the thread count).
+inline void lower_inverse_trans(int m, const double *L, int ldl, double *X, int ldx, int threads = 1) {
+    constexpr int kBlock = 64;
+    const GemmFn gemm = gemm_nt();
+    for (int j0 = 0; j0 < m; j0 += kBlock) {
+        const int j1 = std::min(m, j0 + kBlock);
+        for (int j = j0; j < j1; ++j) {
+            double *xj = X + (size_t)j * ldx;
+            for (int p = j0; p < j; ++p) {
+                const double l = L[j + (size_t)p * ldl];
+                const double *xp = X + (size_t)p * ldx;
+                for (int i = 0; i <= p; ++i) xj[i] -= l * xp[i];
+            }
+            const double d = L[j + (size_t)j * ldl];
+            for (int i = 0; i <= j; ++i) xj[i] /= d;
+        }
+        if (j1 >= m) continue;
+        const int cols = m - j1;
+        const int parts = threads > 1 ? std::min(threads, (cols + 63) / 64) : 1;
+        parallel_parts(parts, parts, [&](int q) {
+            const int c0 = part_bound(cols, parts, q, kNR), c1 = part_bound(cols, parts, q + 1, kNR);
+            if (c1 > c0)
+                gemm(false, j1, c1 - c0, j1 - j0, X + (size_t)j0 * ldx, ldx, L + j1 + c0 + (size_t)j0 * ldl, ldl,
+                     X + (size_t)(j1 + c0) * ldx, ldx);
+        });
+    }
+}
+
+// C(r x m) -= A(r x m) L^{-1}, with L^{-1} given as X = L^{-T} (column-major,
+// as lower_inverse_trans leaves it): entry (p, j) of L^{-1} is X[j + p ldx]
+// and zero for p < j, so the column panel [j0, j0 + w) of C reduces over
+// p >= j0 only (half the work of the full product).
+// threads > 1 splits the rows of C.
+inline void gemm_lower_inverse(int r, int m, const double *A, int lda, const double *X, int ldx, double *C,
+                               int ldc, int threads = 1) {
+    constexpr int kPanel = 64;
+    const GemmFn gemm = gemm_nt();
+    const int parts = threads > 1 ? std::min(threads, (r + 63) / 64) : 1;
+    parallel_parts(parts, parts, [&](int q) {
+        const int r0 = part_bound(r, parts, q, kMR), r1 = part_bound(r, parts, q + 1, kMR);
+        if (r1 <= r0) return;
+        for (int j0 = 0; j0 < m; j0 += kPanel) {
+            const int w = std::min(m, j0 + kPanel) - j0;
+            gemm(false, r1 - r0, w, m - j0, A + r0 + (size_t)j0 * lda, lda, X + j0 + (size_t)j0 * ldx, ldx,
+                 C + r0 + (size_t)j0 * ldc, ldc);
+        }
+    });
 }
 
 }  // namespace dense

@@ -19,21 +19,7 @@ namespace cholespy_amd {
 
 namespace {
 
-// Runs fn(part) for part in [0, parts) on up to `threads` threads.
-void parallel_parts(int parts, int threads, const std::function<void(int)> &fn) {
-    if (threads <= 1 || parts <= 1) {
-        for (int p = 0; p < parts; ++p) fn(p);
-        return;
-    }
-    std::vector<std::thread> pool;
-    const int t = std::min(threads, parts);
-    for (int w = 1; w < t; ++w)
-        pool.emplace_back([&, w] {
-            for (int p = w; p < parts; p += t) fn(p);
-        });
-    for (int p = 0; p < parts; p += t) fn(p);
-    for (auto &th : pool) th.join();
-}
+using dense::parallel_parts;
 
 constexpr int kNB = 64;             // block columns of the partial factorisation
 constexpr int kParallelRows = 768;  // trailing sizes worth splitting over threads
