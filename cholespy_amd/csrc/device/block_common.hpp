@@ -15,7 +15,7 @@ struct BlockPassArgs {
     int ntasks, k;
     int chunk;              // narrow kernels: rows of a vector staged in LDS at a time
     int lds_bytes;          // panel kernels: dynamic LDS of the launch
-    const int *task;        // kTaskInts ints per task (block_plan.hpp)
+    const int *task;        // kDevTaskInts ints per task (DeviceTask in launchers.hpp)
     const int *start;       // block row ranges
     const int *rptr;        // R list / partial slot ranges
     const int *rlist;
@@ -52,9 +52,13 @@ struct VecMap {
     long long vbase, pbase, sbase;
 };
 
-// One task record (block_plan.hpp).
+// One task record as uploaded: the plan's record (block_plan.hpp) and its
+// block's row range, R slots, M_B offset and system (DeviceTask).
 struct Task {
     int blk, first, last, kind, lo, hi, sbase, nsplit;
+    int a0, m, r0, R;
+    int64_t moff;
+    int sys;
 };
 
 // Panel (MFMA) variants of the block passes, trsv_panel.hip.
@@ -70,16 +74,20 @@ constexpr int kTaskThreads = 256;
 constexpr int kGatherUnroll = 8;
 
 template <typename F>
-__device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, int blk) {
+__device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, const Task &task) {
     if (!a.blk_sys) return VecMap{a.k, 0, 0, 0};
-    const int sys = a.blk_sys[blk];
+    const int sys = task.sys;
     return VecMap{a.sys_k[sys], a.sys_vbase[sys], a.sys_pbase[sys], a.sys_sbase[sys]};
 }
 
+// The whole record is one 64-byte line: four 16-byte loads in flight together.
 template <typename F>
 __device__ __forceinline__ Task load_task(const BlockPassArgs<F> &a, int tk) {
-    const int *t = a.task + (size_t)kTaskInts * tk;
-    return Task{t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7]};
+    const int4 *t = reinterpret_cast<const int4 *>(a.task + (size_t)kDevTaskInts * tk);
+    const int4 w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3];
+    return Task{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                w2.x, w2.y, w2.z, w2.w,
+                (int64_t)(((uint64_t)(uint32_t)w3.y << 32) | (uint32_t)w3.x), w3.z};
 }
 
 // Value of a published entry: one agent-scope load, and the bounded wait of

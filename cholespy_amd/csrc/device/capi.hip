@@ -93,7 +93,7 @@ struct DeviceBlockPlan {
 struct cholespy_gpu_solver;
 namespace {
 BlockPlanOptions plan_options();
-int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P);
+int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<int> *blk_sys = nullptr);
 }  // namespace
 
 struct cholespy_gpu_solver {
@@ -183,8 +183,30 @@ BlockPlanOptions plan_options() {
     return opt;
 }
 
-// Copies a (possibly merged) block plan to the device in the handle's precision.
-int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
+// Device task records (kDevTaskInts ints each) for the plan's tasks.
+std::vector<int> device_tasks(const BlockPlan &P, const std::vector<int> &tasks, const std::vector<int> *blk_sys) {
+    const size_t nt = tasks.size() / kTaskInts;
+    std::vector<int> out(nt * kDevTaskInts, 0);
+    for (size_t t = 0; t < nt; ++t) {
+        const int *src = &tasks[t * kTaskInts];
+        int *dst = &out[t * kDevTaskInts];
+        for (int i = 0; i < kTaskInts; ++i) dst[i] = src[i];
+        const int b = src[0];
+        dst[8] = P.start[b];
+        dst[9] = P.start[b + 1] - P.start[b];
+        dst[10] = P.rptr[b];
+        dst[11] = P.rptr[b + 1] - P.rptr[b];
+        const uint64_t mo = (uint64_t)P.moff[b];
+        dst[12] = (int)(uint32_t)(mo & 0xffffffffu);
+        dst[13] = (int)(uint32_t)(mo >> 32);
+        dst[14] = blk_sys ? (*blk_sys)[b] : 0;
+    }
+    return out;
+}
+
+// Copies a (possibly merged) block plan to the device in the handle's
+// precision; blk_sys maps blocks to systems for a batch.
+int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<int> *blk_sys) {
     DeviceBlockPlan &d = s->plan;
     d.release();
     d.nblocks = P.nblocks;
@@ -213,8 +235,11 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P) {
     HIP_TRY(upload(&d.moff, P.moff.data(), P.moff.size()));
     HIP_TRY(upload(&d.cptr, P.contrib_ptr.data(), P.contrib_ptr.size()));
     HIP_TRY(upload(&d.cslot, P.contrib_slot.data(), P.contrib_slot.size()));
-    HIP_TRY(upload(&d.ftask, P.fwd_task.data(), P.fwd_task.size()));
-    HIP_TRY(upload(&d.btask, P.bwd_task.data(), P.bwd_task.size()));
+    {
+        const std::vector<int> ft = device_tasks(P, P.fwd_task, blk_sys), bt = device_tasks(P, P.bwd_task, blk_sys);
+        HIP_TRY(upload(&d.ftask, ft.data(), ft.size()));
+        HIP_TRY(upload(&d.btask, bt.data(), bt.size()));
+    }
     const size_t me = (size_t)d.m_entries;
     if (s->fbytes == 4) {
         std::vector<float> f(me), b(me);
@@ -998,7 +1023,7 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     emit(btasks, false, G.bwd_task);
     plans.clear();
     HIP_TRY(hipMemcpy(s->perm, gperm.data(), (size_t)s->n * sizeof(int), hipMemcpyHostToDevice));
-    int rc = upload_plan(s, G);
+    int rc = upload_plan(s, G, &blk_sys);
     if (rc != CHOLESPY_OK) return rc;
     dev_free(s->blk_sys);
     HIP_TRY(upload(&s->blk_sys, blk_sys.data(), blk_sys.size()));

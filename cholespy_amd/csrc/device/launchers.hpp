@@ -71,11 +71,18 @@ inline int block_chunk(int float_bytes, int tile) { return kBlockLdsBytes / (til
 // Dynamic LDS of a panel launch (override: CHOLESPY_PANEL_LDS, 16-64 KiB);
 // at least 16 KiB, the backward kernel's staged R-row indices.
 constexpr int kPanelLdsBytes = 32 * 1024;
+// Task records as uploaded (kDevTaskInts ints, one 64-byte line): the
+// plan's kTaskInts ints (block, first, last, kind, lo, hi, sbase, split),
+// then the block's first row a0, size m, first partial slot r0, |R|, the
+// M_B offset (low and high 32 bits), its system (0 for a single system)
+// and one unused int. A task needs one line of metadata, not a second round
+// of loads indexed by its block.
+constexpr int kDevTaskInts = 16;
 struct BlockPassBuffers {
     int ntasks = 0, k = 0;
     int max_m = 0, max_rows = 0;   // largest block (m) and M_B height (m + |R|)
     int max_k = 0;                 // largest right-hand-side count of any system (tile choice)
-    const int *task = nullptr;     // (block, first, last, kind) per task, in ticket order
+    const int *task = nullptr;     // kDevTaskInts ints per task, in ticket order
     const int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
     const int64_t *moff = nullptr;
     const void *M = nullptr;       // forward or backward layout of M_B

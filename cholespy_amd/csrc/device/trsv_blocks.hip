@@ -40,6 +40,12 @@ constexpr int kBLanes = 32;     // backward: most columns of M_B per task
 #define CHOLESPY_KMU 16
 #endif
 constexpr int kMU = CHOLESPY_KMU;   // M loads issued together per thread
+#ifndef CHOLESPY_KPRE
+#define CHOLESPY_KPRE 8
+#endif
+// M entries per thread loaded before the task's right-hand-side gather, which
+// may wait on other tasks: their latency overlaps the wait and the gather.
+constexpr int kPre = CHOLESPY_KPRE;
 
 template <typename F, int KT>
 __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a) {
@@ -54,17 +60,15 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
     while (tk >= 0) {
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
-        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
+        const int first = task.first, last = task.last, kind = task.kind;
         // Lane layout of the product: `lanes` rows x `segs` column segments.
         const int lanes = last - first <= 16 ? 16 : (last - first <= 32 ? 32 : kLanes);
         const int segs = kTaskThreads / lanes;
         const int lr = tid & (lanes - 1), seg = tid / lanes;
-        const int a0 = a.start[blk];
-        const int m = a.start[blk + 1] - a0;
-        const int R = a.rptr[blk + 1] - a.rptr[blk];
+        const int a0 = task.a0, m = task.m, R = task.R;
         const int rows = m + R;
-        const F *M = a.M + a.moff[blk];
-        const VecMap vm = vec_map(a, blk);
+        const F *M = a.M + task.moff;
+        const VecMap vm = vec_map(a, task);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
         const F *bv = a.b + vm.vbase;
@@ -77,7 +81,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                 yv[o] = v;
                 zv[o] = from_bits<F>(Bits<F>::sentinel);
             } else {
-                publish(&pv[(long long)(a.rptr[blk] + r - m) * k + t], v);
+                publish(&pv[(long long)(task.r0 + r - m) * k + t], v);
             }
         };
         // Reduction range: all columns, or [lo, hi) for a split task, whose
@@ -85,6 +89,28 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const int clo = kind == 3 ? task.lo : 0, chi = kind == 3 ? task.hi : m;
         const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
         const int srow = task.sbase + sidx * (last - first) - first;
+        // This thread's columns of the first chunk (see the product below);
+        // the first kPre of them are loaded before the gather.
+        auto cols_of = [&](int c0, int c1, int &cb, int &ce) {
+            const int cuse = min(c1, min(last, m));
+            const int cend = min(rho < m ? rho + 1 : m, c1);
+            const int cq = max(0, (cuse - c0 + segs - 1) / segs);
+            cb = c0 + seg * cq;
+            ce = min(cend, cb + cq);
+        };
+        F mpre[kPre];
+        int npre = 0;
+        if (kind != 1 && rho < last) {
+            int cb, ce;
+            cols_of(clo, min(chi, clo + a.chunk), cb, ce);
+            npre = max(0, min(kPre, ce - cb));
+            int64_t o = npre > 0 ? fwd_off(rows, cb, rho) : 0;
+#pragma unroll
+            for (int j = 0; j < kPre; ++j) {
+                mpre[j] = j < npre ? M[o] : F(0);
+                o += rows - (cb + j) - 1;
+            }
+        }
         for (int t0 = 0; t0 < k; t0 += KT) {
             const int kt = min(KT, k - t0);
             if (kind == 1) {   // gather-only task: rows [first, last) of r_B to rbuf
@@ -111,15 +137,25 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     // The columns the task's rows use, [c0, min(c1, cuse)), split
                     // evenly over the segments (all lanes of a wavefront load the
                     // same column: one coalesced access); row rho uses c < cend.
-                    const int cuse = min(c1, min(last, m));
-                    const int cend = min(rho < m ? rho + 1 : m, c1);
-                    const int cq = max(0, (cuse - c0 + segs - 1) / segs);
-                    const int cb = c0 + seg * cq, ce = min(cend, cb + cq);
+                    int cb, ce;
+                    cols_of(c0, c1, cb, ce);
+                    int c = cb;
+                    if (t0 == 0 && c0 == clo && npre > 0) {   // the prefetched columns
+#pragma unroll
+                        for (int j = 0; j < kPre; ++j) {
+                            if (j < npre) {
+                                const F *r = rv + (size_t)(cb + j - c0) * KT;
+#pragma unroll
+                                for (int t = 0; t < KT; ++t) acc[t] = fma(mpre[j], r[t], acc[t]);
+                            }
+                        }
+                        c += npre;
+                    }
                     // Column c of the packed column-major M_B starts (rows - c)
                     // entries after column c - 1: walk the offsets incrementally,
                     // kMU loads in flight per batch (the last batch predicated).
-                    int64_t o = cb < ce ? fwd_off(rows, cb, rho) : 0;
-                    for (int c = cb; c < ce; c += kMU) {
+                    int64_t o = c < ce ? fwd_off(rows, c, rho) : 0;
+                    for (; c < ce; c += kMU) {
                         F mv[kMU];
 #pragma unroll
                         for (int j = 0; j < kMU; ++j) {
@@ -184,18 +220,15 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
     while (tk >= 0) {
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
-        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
+        const int first = task.first, last = task.last, kind = task.kind;
         // Lane layout of the product: `lanes` columns x `segs` row segments.
         const int lanes = last - first <= 16 ? 16 : kBLanes;
         const int segs = kTaskThreads / lanes;
         const int lr = tid & (lanes - 1), seg = tid / lanes;
-        const int a0 = a.start[blk];
-        const int m = a.start[blk + 1] - a0;
-        const int r0 = a.rptr[blk];
-        const int R = a.rptr[blk + 1] - r0;
+        const int a0 = task.a0, m = task.m, r0 = task.r0, R = task.R;
         const int rows = m + R;
-        const F *M = a.M + a.moff[blk];
-        const VecMap vm = vec_map(a, blk);
+        const F *M = a.M + task.moff;
+        const VecMap vm = vec_map(a, task);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
         F *sp = a.spart + vm.sbase;
@@ -216,6 +249,27 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         const int hlo = kind == 1 ? task.lo : 0, hhi = kind == 1 ? task.hi : rows;
         const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
         const int srow = task.sbase + sidx * (last - first) - first;
+        // This thread's rows of a chunk (see the product below); the first
+        // kPre of the first chunk are loaded before the gather.
+        auto rows_of = [&](int h0, int h1, int &rb, int &re) {
+            const int hb = max(h0, first);
+            const int rq = max(0, (h1 - hb + segs - 1) / segs);
+            rb = max(hb + seg * rq, c);
+            re = min(h1, hb + seg * rq + rq);
+        };
+        F mpre[kPre];
+        int npre = 0;
+        if (c < last) {
+            int rb, re;
+            rows_of(hlo, min(hhi, hlo + a.chunk), rb, re);
+            npre = max(0, min(kPre, re - rb));
+            int64_t o = npre > 0 ? bwd_off(m, rb, c) : 0;
+#pragma unroll
+            for (int j = 0; j < kPre; ++j) {
+                mpre[j] = j < npre ? M[o] : F(0);
+                o += min(rb + j + 1, m);
+            }
+        }
         for (int t0 = 0; t0 < k; t0 += KT) {
             const int kt = min(KT, k - t0);
             F acc[KT];
@@ -238,11 +292,22 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     // incrementally, kMU loads in flight per batch.
                     // Rows [max(h0, first), h1) split evenly over the segments
                     // (rows below the task's first column are zero in its columns).
-                    const int hb = max(h0, first);
-                    const int rq = max(0, (h1 - hb + segs - 1) / segs);
-                    const int rb = max(hb + seg * rq, c), re = min(h1, hb + seg * rq + rq);
-                    int64_t o = rb < re ? bwd_off(m, rb, c) : 0;
-                    for (int rho = rb; rho < re; rho += kMU) {
+                    int rb, re;
+                    rows_of(h0, h1, rb, re);
+                    int rho = rb;
+                    if (t0 == 0 && h0 == hlo && npre > 0) {   // the prefetched rows
+#pragma unroll
+                        for (int j = 0; j < kPre; ++j) {
+                            if (j < npre) {
+                                const F *v = vs + (size_t)(rb + j - h0) * KT;
+#pragma unroll
+                                for (int t = 0; t < KT; ++t) acc[t] = fma(mpre[j], v[t], acc[t]);
+                            }
+                        }
+                        rho += npre;
+                    }
+                    int64_t o = rho < re ? bwd_off(m, rho, c) : 0;
+                    for (; rho < re; rho += kMU) {
                         F mv[kMU];
 #pragma unroll
                         for (int j = 0; j < kMU; ++j) {

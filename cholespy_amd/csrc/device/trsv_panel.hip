@@ -247,12 +247,11 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
     while (tk >= 0) {
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
-        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
-        const int a0 = a.start[blk];
-        const int m = a.start[blk + 1] - a0;
-        const int rows = m + (a.rptr[blk + 1] - a.rptr[blk]);
-        const F *M = a.M + a.moff[blk];
-        const VecMap vm = vec_map(a, blk);
+        const int first = task.first, last = task.last, kind = task.kind;
+        const int a0 = task.a0, m = task.m;
+        const int rows = m + task.R;
+        const F *M = a.M + task.moff;
+        const VecMap vm = vec_map(a, task);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase, *rb = a.rbuf + vm.vbase;
         F *sp = a.spart + vm.sbase;
@@ -264,7 +263,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
                 yv[o] = v;
                 zv[o] = from_bits<F>(Bits<F>::sentinel);
             } else {
-                publish(&pv[(long long)(a.rptr[blk] + r - m) * k + t], v);
+                publish(&pv[(long long)(task.r0 + r - m) * k + t], v);
             }
         };
         const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
@@ -411,14 +410,11 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
     while (tk >= 0) {
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
         const Task task = load_task(a, tk);
-        const int blk = task.blk, first = task.first, last = task.last, kind = task.kind;
-        const int a0 = a.start[blk];
-        const int m = a.start[blk + 1] - a0;
-        const int r0 = a.rptr[blk];
-        const int R = a.rptr[blk + 1] - r0;
+        const int first = task.first, last = task.last, kind = task.kind;
+        const int a0 = task.a0, m = task.m, r0 = task.r0, R = task.R;
         const int rows = m + R;
-        const F *M = a.M + a.moff[blk];
-        const VecMap vm = vec_map(a, blk);
+        const F *M = a.M + task.moff;
+        const VecMap vm = vec_map(a, task);
         const int k = vm.k;
         F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
         F *sp = a.spart + vm.sbase;
