@@ -150,10 +150,16 @@ __device__ __forceinline__ int take_ticket(const QueueArgs &qa, int q, int n) {
 }
 
 // Called once per workgroup by one thread after its last ticket: the last
-// workgroup of the grid zeroes the counters for the next launch.
+// workgroup of the grid zeroes the counters for the next launch. Relaxed is
+// enough: a workgroup gets here only after its last ticket's fetch-add has
+// returned, so when the last one sees the full count no fetch-add on the
+// counters is pending, and the launch boundary orders the zeroing stores
+// before the next launch. (An acquire-release here would write back and
+// invalidate the workgroup's L2 at every exit, under the tasks still
+// running.)
 __device__ __forceinline__ void finish_queues(const QueueArgs &qa) {
     unsigned int *done = qa.ctr + (size_t)kMaxQueues * kQueueStride;
-    const unsigned int d = __hip_atomic_fetch_add(done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned int d = __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (d + 1 == gridDim.x * gridDim.y * gridDim.z) {
         for (int q = 0; q < qa.nq; ++q)
             __hip_atomic_store(qa.ctr + (size_t)q * kQueueStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
