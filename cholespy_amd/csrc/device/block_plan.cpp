@@ -224,6 +224,27 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
         lo = hi;
     }
     P.nsplit_rows = std::max(fwd_split_rows, next_split);
+
+    // 6. Leaf blocks' tasks first (forward) and last (backward), otherwise in
+    //    the same order.
+    auto is_leaf = [&](int b) {
+        const int m = start[b + 1] - start[b], R = P.rptr[b + 1] - P.rptr[b];
+        return P.level_fwd[b] == 0 && m <= kLeafMaxM && m + R <= kLeafMaxRows && m < opt.split_gather &&
+               opt.fwd_rows <= 64 && opt.bwd_cols <= 32;
+    };
+    auto partition = [&](std::vector<int> &tasks, bool leaf_first, int &count) {
+        std::vector<int> leaf, rest;
+        const size_t nt = tasks.size() / kTaskInts;
+        for (size_t t = 0; t < nt; ++t) {
+            std::vector<int> &dst = is_leaf(tasks[t * kTaskInts]) ? leaf : rest;
+            dst.insert(dst.end(), tasks.begin() + t * kTaskInts, tasks.begin() + (t + 1) * kTaskInts);
+        }
+        count = (int)(leaf.size() / kTaskInts);
+        tasks = leaf_first ? leaf : rest;
+        tasks.insert(tasks.end(), leaf_first ? rest.begin() : leaf.begin(), leaf_first ? rest.end() : leaf.end());
+    };
+    partition(P.fwd_task, true, P.n_leaf_fwd);
+    partition(P.bwd_task, false, P.n_leaf_bwd);
     return P;
 }
 

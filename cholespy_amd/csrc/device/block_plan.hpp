@@ -20,6 +20,13 @@
 namespace cholespy_amd {
 
 constexpr int kTaskInts = 8;   // ints per task record (CHOLESPY_TASK_INTS of the C ABI)
+// Leaf blocks: blocks without child blocks (forward level 0) of at most
+// kLeafMaxM rows and kLeafMaxRows rows of M_B. They take nothing from other
+// blocks in the forward pass and give nothing to other blocks in the
+// backward pass, so their tasks can run in a launch of their own before the
+// forward pass and after the backward pass (one wavefront per task).
+constexpr int kLeafMaxM = 64;
+constexpr int kLeafMaxRows = 160;
 // Split index and split count of a split task, in one int.
 constexpr int split_code(int s, int nsplit) { return nsplit | (s << 8); }
 constexpr int split_count(int code) { return code & 255; }
@@ -35,7 +42,7 @@ struct BlockPlanOptions {
     // tree, where few tasks run at once: their product tasks are narrower so
     // more workgroups stream each block's M_B concurrently.
     int fwd_rows_big = 32;  // rows per forward product task of such blocks
-    int bwd_cols_big = 16;  // columns per backward task of such blocks
+    int bwd_cols_big = 32;  // columns per backward task of such blocks
     // A product task of such a block that covers more than split_entries
     // entries of M_B is split along its reduction (columns forward, rows
     // backward) into up to max_split tasks of about split_entries entries
@@ -77,6 +84,10 @@ struct BlockPlan {
     // Backward kinds: 0 = columns [first, last) of M_B; 1 = split s of
     // nsplit over rows [lo, hi), combined the same way and finished as 0.
     std::vector<int> fwd_task, bwd_task;
+    // The leaf blocks' tasks are the first n_leaf_fwd forward tasks and the
+    // last n_leaf_bwd backward tasks (the order stays valid for one launch
+    // over all tasks).
+    int n_leaf_fwd = 0, n_leaf_bwd = 0;
     int nsplit_rows = 0;                   // rows of the split-result array
     int max_m = 0, max_rows = 0;           // largest m and m + |R|
     int64_t nnz_l = 0;

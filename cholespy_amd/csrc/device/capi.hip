@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <thread>
 #include <vector>
@@ -66,6 +67,7 @@ hipError_t upload(T **dst, const T *src, size_t count) {
 struct DeviceBlockPlan {
     int nblocks = 0, nslots = 0, nsplit_rows = 0, max_m = 0, max_rows = 0;
     int n_fwd_tasks = 0, n_bwd_tasks = 0;
+    int n_leaf_fwd = 0, n_leaf_bwd = 0;   // leaf tasks: forward prefix, backward suffix
     int levels_fwd = 0, levels_bwd = 0;
     int64_t m_entries = 0;
     int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
@@ -122,6 +124,7 @@ struct cholespy_gpu_solver {
     void *rbuf = nullptr;                 // block path gathered right-hand sides (n x k)
     int k_cur = 0;                        // layout the work buffers hold
     int path = CHOLESPY_PATH_BLOCKS;
+    bool leaf_kernels = true;             // leaf tasks in launches of their own (CHOLESPY_LEAF_KERNELS=0: off)
     int buffers_path = -1;                // path whose invariants the buffers hold
     size_t work_elems = 0;                // capacity of y and z in elements
     size_t part_elems = 0;
@@ -219,6 +222,8 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
     d.max_rows = P.max_rows;
     d.n_fwd_tasks = (int)(P.fwd_task.size() / kTaskInts);
     d.n_bwd_tasks = (int)(P.bwd_task.size() / kTaskInts);
+    d.n_leaf_fwd = P.n_leaf_fwd;
+    d.n_leaf_bwd = P.n_leaf_bwd;
     d.levels_fwd = d.levels_bwd = 0;
     for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
     for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
@@ -366,6 +371,7 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
         // Block-pass ticket queues (tuning sweeps: CHOLESPY_BLOCK_QUEUES, 1..16).
         if (const char *e = std::getenv("CHOLESPY_BLOCK_QUEUES"))
             s->queues[4].nq = s->queues[5].nq = std::min(16, std::max(1, std::atoi(e)));
+    if (const char *e = std::getenv("CHOLESPY_LEAF_KERNELS")) s->leaf_kernels = std::atoi(e) != 0;
     if (e != hipSuccess) {
         cholespy_solver_destroy(s);
         return hip_fail(e, "cholespy_solver_create");
@@ -711,6 +717,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
         if (only_pass >= 0 && pass != only_pass) continue;
         const bool forward = pass == 0;
         bf.ntasks = forward ? d.n_fwd_tasks : d.n_bwd_tasks;
+        bf.n_leaf = s->leaf_kernels ? (forward ? d.n_leaf_fwd : d.n_leaf_bwd) : 0;
         bf.task = forward ? d.ftask : d.btask;
         bf.M = forward ? d.mfwd : d.mbwd;
         bf.trace = s->trace[forward ? 1 : 0];
@@ -986,10 +993,17 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         G.mbwd.insert(G.mbwd.end(), P.mbwd.begin(), P.mbwd.end());
         for (int r = 0; r < s->sys_n[i]; ++r) G.contrib_ptr.push_back(P.contrib_ptr[r] + C0);
         for (int v : P.contrib_slot) G.contrib_slot.push_back(v + S0);
-        for (size_t t = 0; t < P.fwd_task.size() / kTaskInts; ++t)
-            ftasks.push_back({P.level_fwd[P.fwd_task[kTaskInts * t]], i, (int)t});
-        for (size_t t = 0; t < P.bwd_task.size() / kTaskInts; ++t)
-            btasks.push_back({P.level_bwd[P.bwd_task[kTaskInts * t]], i, (int)t});
+        // Sort keys: the level, with leaf tasks first (forward) or last
+        // (backward) as in each system's plan.
+        const size_t nf = P.fwd_task.size() / kTaskInts, nb = P.bwd_task.size() / kTaskInts;
+        for (size_t t = 0; t < nf; ++t)
+            ftasks.push_back({(int)t < P.n_leaf_fwd ? -1 : P.level_fwd[P.fwd_task[kTaskInts * t]], i, (int)t});
+        for (size_t t = 0; t < nb; ++t)
+            btasks.push_back({t >= nb - (size_t)P.n_leaf_bwd ? std::numeric_limits<int>::max()
+                                                             : P.level_bwd[P.bwd_task[kTaskInts * t]],
+                              i, (int)t});
+        G.n_leaf_fwd += P.n_leaf_fwd;
+        G.n_leaf_bwd += P.n_leaf_bwd;
         G.max_m = std::max(G.max_m, P.max_m);
         G.max_rows = std::max(G.max_rows, P.max_rows);
         G.nnz_l += P.nnz_l;

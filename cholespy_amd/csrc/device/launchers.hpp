@@ -80,6 +80,9 @@ constexpr int kPanelLdsBytes = 32 * 1024;
 constexpr int kDevTaskInts = 16;
 struct BlockPassBuffers {
     int ntasks = 0, k = 0;
+    // Leaf tasks of this pass (forward: the first n_leaf tasks, backward: the
+    // last), run by the leaf kernels on the narrow path.
+    int n_leaf = 0;
     int max_m = 0, max_rows = 0;   // largest block (m) and M_B height (m + |R|)
     int max_k = 0;                 // largest right-hand-side count of any system (tile choice)
     const int *task = nullptr;     // kDevTaskInts ints per task, in ticket order

@@ -367,6 +367,183 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
     if (tid == 0) finish_queues(a.qa);
 }
 
+// Leaf blocks (block_plan.hpp: no child blocks, at most kLeafMaxM rows and
+// kLeafMaxRows rows of M_B): one wavefront per task, kLeafWaves tasks per
+// workgroup, no tickets and no waits. The forward launch runs before the
+// forward block pass (leaf rows receive no partials, so r_B = (P b)_B); the
+// backward launch runs after the backward block pass, when every z_R is
+// final. The tasks and the results are those of the block kernels; only the
+// lane layout and the reduction order inside a task differ.
+constexpr int kLeafWaves = kTaskThreads / kWave;
+#ifndef CHOLESPY_KLEAFMU
+#define CHOLESPY_KLEAFMU 16
+#endif
+constexpr int kLeafMU = CHOLESPY_KLEAFMU;   // M loads in flight per lane
+
+// Orders this wavefront's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename F, int KT>
+__global__ __launch_bounds__(kTaskThreads) void leaf_forward(BlockPassArgs<F> a) {
+    __shared__ F rs[kLeafWaves][kLeafMaxM * KT];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tk = blockIdx.x * kLeafWaves + wave;
+    if (tk >= a.ntasks) return;
+    if (a.trace && lane == 0) a.trace[4 * (size_t)tk] = wall_clock64();
+    const Task task = load_task(a, tk);
+    const int first = task.first, last = task.last, a0 = task.a0, m = task.m;
+    const int rows = m + task.R;
+    const F *M = a.M + task.moff;
+    const VecMap vm = vec_map(a, task);
+    const int k = vm.k;
+    // Lane = row rho of M_B; row rho uses columns [0, cend).
+    const int rho = first + lane;
+    const int cend = rho < last ? min(rho + 1, m) : 0;
+    F mv[kLeafMU];
+    int64_t o = fwd_off(rows, 0, rho);
+#pragma unroll
+    for (int j = 0; j < kLeafMU; ++j) {
+        mv[j] = j < cend ? M[o] : F(0);
+        o += rows - j - 1;
+    }
+    // r_B = (P b)_B, rows [0, m).
+    F *r = rs[wave];
+    if (lane < m) {
+        const long long pr = vm.vbase + (long long)a.perm[a0 + lane] * k;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) r[lane * KT + t] = t < k ? a.b[pr + t] : F(0);
+    }
+    wave_sync();
+    if (a.trace && lane == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+    F acc[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] = F(0);
+    for (int c = 0;;) {
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            if (c + j < cend) {
+                const F *rr = r + (c + j) * KT;
+#pragma unroll
+                for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], rr[t], acc[t]);
+            }
+        }
+        c += kLeafMU;
+        if (c >= cend) break;
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            mv[j] = c + j < cend ? M[o] : F(0);
+            o += rows - (c + j) - 1;
+        }
+    }
+    if (rho < last) {
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            if (t >= k) break;
+            if (rho < m) {
+                const long long q = (long long)(a0 + rho) * k + t;
+                yv[q] = acc[t];
+                zv[q] = from_bits<F>(Bits<F>::sentinel);
+            } else {
+                publish(&pv[(long long)(task.r0 + rho - m) * k + t], acc[t]);
+            }
+        }
+    }
+    if (a.trace && lane == 0) {
+        a.trace[4 * (size_t)tk + 2] = wall_clock64();
+        a.trace[4 * (size_t)tk + 3] = blockIdx.x;
+    }
+}
+
+template <typename F, int KT>
+__global__ __launch_bounds__(kTaskThreads) void leaf_backward(BlockPassArgs<F> a) {
+    __shared__ F vsh[kLeafWaves][kLeafMaxRows * KT];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int tk = blockIdx.x * kLeafWaves + wave;
+    if (tk >= a.ntasks) return;
+    if (a.trace && lane == 0) a.trace[4 * (size_t)tk] = wall_clock64();
+    const Task task = load_task(a, tk);
+    const int first = task.first, last = task.last, a0 = task.a0, m = task.m, r0 = task.r0, R = task.R;
+    const int rows = m + R;
+    const F *M = a.M + task.moff;
+    const VecMap vm = vec_map(a, task);
+    const int k = vm.k;
+    F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+    // Lanes l and l + 32 share column c = first + l % 32 of M_B and take the
+    // rows rho = first + l / 32 + 2 j, rho >= c (rows of one step are
+    // consecutive entries of the row-major packed M_B across the lanes).
+    const int c = first + (lane & 31), h = lane >> 5;
+    const bool active = c < last;
+    F mv[kLeafMU];
+#pragma unroll
+    for (int j = 0; j < kLeafMU; ++j) {
+        const int rr = first + h + 2 * j;
+        mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
+    }
+    // v = [y_B ; z_R], rows [first, rows): y from the forward pass, z_R from
+    // the backward block pass (both complete before this launch).
+    F *v = vsh[wave];
+    for (int i = first + lane; i < rows; i += kWave) {
+        const F *src = i < m ? yv + (long long)(a0 + i) * k : zv + (long long)a.rlist[r0 + i - m] * k;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) v[(i - first) * KT + t] = t < k ? src[t] : F(0);
+    }
+    wave_sync();
+    if (a.trace && lane == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
+    F acc[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] = F(0);
+    for (int j0 = 0;; j0 += kLeafMU) {
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            const int rr = first + h + 2 * (j0 + j);
+            if (active && rr >= c && rr < rows) {
+                const F *vv = v + (rr - first) * KT;
+#pragma unroll
+                for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], vv[t], acc[t]);
+            }
+        }
+        if (first + h + 2 * (j0 + kLeafMU) >= rows) break;
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            const int rr = first + h + 2 * (j0 + kLeafMU + j);
+            mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] += __shfl_xor(acc[t], 32, kWave);
+    if (h == 0 && active) {
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            if (t >= k) break;
+            publish(&zv[(long long)(a0 + c) * k + t], acc[t]);
+            const long long q = vm.vbase + (long long)a.perm[a0 + c] * k + t;
+            if (a.xmode == 0) {
+                static_cast<F *>(a.x)[q] = acc[t];
+            } else {
+                double *x64 = static_cast<double *>(a.x);
+                x64[q] = (a.xmode == 2 ? x64[q] : 0.0) + (double)acc[t];
+            }
+        }
+    }
+    // Restore the sentinel in the block's partial slots and r_B rows for the
+    // next solve (once per block: its first column task).
+    if (first == 0) {
+        const F sent = from_bits<F>(Bits<F>::sentinel);
+        for (int idx = lane; idx < R * k; idx += kWave) pv[(long long)r0 * k + idx] = sent;
+        F *rb = a.rbuf + vm.vbase;
+        for (int idx = lane; idx < m * k; idx += kWave) rb[(long long)a0 * k + idx] = sent;
+    }
+    if (a.trace && lane == 0) {
+        a.trace[4 * (size_t)tk + 2] = wall_clock64();
+        a.trace[4 * (size_t)tk + 3] = blockIdx.x;
+    }
+}
+
 template <typename F>
 hipError_t launch_narrow(bool forward, const BlockPassArgs<F> &a, int tile, size_t lds, int grid,
                          hipStream_t stream) {
@@ -416,7 +593,36 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
         else return launch_panel_f64(forward, a, tile / 16, grid, stream);
     }
     const size_t lds = block_pass_lds(sizeof(F), forward ? bf.max_m : bf.max_rows, tile);
-    return launch_narrow<F>(forward, a, tile, lds, grid, stream);
+    // Leaf tasks (forward: the first n_leaf, backward: the last n_leaf) in
+    // their own launch, before / after the block pass over the others.
+    const int n_leaf = std::min(bf.n_leaf, bf.ntasks);
+    const int n_main = bf.ntasks - n_leaf;
+    auto leaf = [&](int offset) -> hipError_t {
+        BlockPassArgs<F> l = a;
+        l.task = a.task + (size_t)offset * kDevTaskInts;
+        l.ntasks = n_leaf;
+        if (a.trace) l.trace = a.trace + 4 * (size_t)offset;
+        void *p[] = {&l};
+        const void *kern = forward ? reinterpret_cast<const void *>(&leaf_forward<F, kNarrowTile>)
+                                   : reinterpret_cast<const void *>(&leaf_backward<F, kNarrowTile>);
+        return hipLaunchKernel(kern, dim3((n_leaf + kLeafWaves - 1) / kLeafWaves), dim3(kTaskThreads), p, 0, stream);
+    };
+    auto main = [&](int offset) -> hipError_t {
+        BlockPassArgs<F> b = a;
+        b.task = a.task + (size_t)offset * kDevTaskInts;
+        b.ntasks = n_main;
+        if (a.trace) b.trace = a.trace + 4 * (size_t)offset;
+        return launch_narrow<F>(forward, b, tile, lds, std::min(grid, n_main), stream);
+    };
+    hipError_t e = hipSuccess;
+    if (forward) {
+        if (n_leaf > 0) e = leaf(0);
+        if (e == hipSuccess && n_main > 0) e = main(n_leaf);
+    } else {
+        if (n_main > 0) e = main(0);
+        if (e == hipSuccess && n_leaf > 0) e = leaf(n_main);
+    }
+    return e;
 }
 
 }  // namespace
