@@ -130,7 +130,7 @@ __device__ __forceinline__ void combine_splits(const BlockPassArgs<F> &a, F *sp,
         const long long stride = (long long)w * k;
         U u[kMaxSplit];
 #pragma unroll
-        for (int s = 0; s < kMaxSplit; ++s) u[s] = s < nsplit ? load_bits(p0 + s * stride) : U(0);
+        for (int s = 0; s < kMaxSplit; ++s) u[s] = s < nsplit ? peek_bits(p0 + s * stride) : U(0);
         F v = F(0);
 #pragma unroll
         for (int s = 0; s < kMaxSplit; ++s)
@@ -220,7 +220,7 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
                 for (int q = 0; q < kPairs; ++q)
 #pragma unroll
                     for (int j = 0; j < kValBatch; ++j)
-                        u[q][j] = sl[q][h + j] >= 0 ? load_bits(&pv[(long long)sl[q][h + j] * k + col[q]]) : U(0);
+                        u[q][j] = sl[q][h + j] >= 0 ? peek_bits(&pv[(long long)sl[q][h + j] * k + col[q]]) : U(0);
 #pragma unroll
                 for (int q = 0; q < kPairs; ++q)
 #pragma unroll
@@ -269,7 +269,7 @@ __device__ __forceinline__ void gather_rows(const BlockPassArgs<F> &a, F *dst, i
             pub[j] = false;
             if (idx < total && t < kt) {
                 ad[j] = at(r0 + i, t, pub[j]);
-                u[j] = pub[j] ? load_bits(ad[j]) : as_bits(*ad[j]);
+                u[j] = pub[j] ? peek_bits(ad[j]) : as_bits(*ad[j]);
             }
         }
 #pragma unroll

@@ -24,6 +24,183 @@ inline int64_t bwd_off(int64_t m, int64_t rho, int64_t c) {      // row-major, c
 
 }  // namespace
 
+namespace {
+
+struct TaskShape {
+    int fwd_rows, bwd_cols, split_gather, fwd_rows_big, bwd_cols_big, split_entries, max_split;
+    int level_tasks, min_split_entries;
+    bool leaf_partition;
+};
+
+struct TaskList {
+    std::vector<int> fwd, bwd;
+    int nsplit_rows = 0, n_leaf_fwd = 0, n_leaf_bwd = 0;
+};
+
+TaskShape narrow_shape(const BlockPlanOptions &o) {
+    return TaskShape{o.fwd_rows, o.bwd_cols, o.split_gather, o.fwd_rows_big, o.bwd_cols_big,
+                     o.split_entries, o.max_split, o.level_tasks, o.min_split_entries, true};
+}
+TaskShape wide_shape(const BlockPlanOptions &o) {
+    return TaskShape{o.wide_fwd_rows, o.bwd_cols, o.wide_split_gather, o.fwd_rows_big, o.bwd_cols_big,
+                     o.split_entries, o.max_split, o.wide_level_tasks, o.wide_min_split_entries, false};
+}
+
+// Tasks of both passes in level order (stable in block index) for one task shape.
+TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
+    TaskList L;
+    const int nb = P.nblocks;
+    const std::vector<int> &start = P.start;
+    int next_split = 0;   // split rows: their own array, numbered per list
+    auto push = [](std::vector<int> &v, int b, int f, int l, int kind, int lo = 0, int hi = 0, int sb = 0,
+                   int ns = 0) {
+        const int rec[kTaskInts] = {b, f, l, kind, lo, hi, sb, ns};
+        v.insert(v.end(), rec, rec + kTaskInts);
+    };
+    // Cuts [lo, hi) into at most `parts` ranges of about equal weight.
+    auto cut = [](int lo, int hi, int parts, const std::function<int64_t(int)> &w) {
+        int64_t total = 0;
+        for (int x = lo; x < hi; ++x) total += w(x);
+        std::vector<int> b{lo};
+        int64_t acc = 0;
+        for (int x = lo; x < hi; ++x) {
+            acc += w(x);
+            if ((int)b.size() < parts && acc * parts >= total * (int64_t)b.size() && x + 1 < hi) b.push_back(x + 1);
+        }
+        b.push_back(hi);
+        return b;
+    };
+    // Entries per split task at a level: split_entries, or finer (down to
+    // min_split_entries) when the level's large blocks would otherwise give
+    // fewer than level_tasks tasks.
+    int level_split = opt.split_entries;
+    auto nsplits = [&](int64_t entries) {
+        if (level_split <= 0 || entries <= level_split) return 1;
+        return (int)std::min<int64_t>(opt.max_split, (entries + level_split - 1) / level_split);
+    };
+    auto set_level_split = [&](const std::vector<int> &order, size_t lo, size_t hi) {
+        level_split = opt.split_entries;
+        if (opt.level_tasks <= 0 || opt.split_entries <= 0) return;
+        int64_t entries = 0;
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q];
+            const int64_t m = start[b + 1] - start[b], R = P.rptr[b + 1] - P.rptr[b];
+            if (m >= opt.split_gather) entries += m * (m + 1) / 2 + R * m;
+        }
+        const int64_t want = entries / opt.level_tasks;
+        level_split = (int)std::max<int64_t>(opt.min_split_entries, std::min<int64_t>(opt.split_entries, want));
+    };
+    std::vector<int> order(nb);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
+    for (size_t lo = 0; lo < order.size();) {
+        // One level at a time: the gather tasks of its large blocks first,
+        // then the product tasks, so every task's inputs come from tasks with
+        // smaller tickets.
+        size_t hi = lo;
+        while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
+        set_level_split(order, lo, hi);
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q], m = start[b + 1] - start[b];
+            if (m < opt.split_gather) continue;
+            for (int r0 = 0; r0 < m; r0 += opt.fwd_rows) push(L.fwd, b, r0, std::min(m, r0 + opt.fwd_rows), 1);
+        }
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q], m = start[b + 1] - start[b];
+            const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
+            if (m < opt.split_gather) {
+                for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows)
+                    push(L.fwd, b, r0, std::min(rows, r0 + opt.fwd_rows), 0);
+                continue;
+            }
+            for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows_big) {
+                const int r1 = std::min(rows, r0 + opt.fwd_rows_big), w = r1 - r0;
+                // Column c feeds the task rows rho >= c (rho < m) and all rows >= m.
+                auto weight = [&](int c) -> int64_t { return r1 - std::max(r0, std::min(c, m)); };
+                const int cuse = std::min(r1, m);
+                int64_t entries = 0;
+                for (int c = 0; c < cuse; ++c) entries += weight(c);
+                const int ns = nsplits(entries);
+                if (ns == 1) {
+                    push(L.fwd, b, r0, r1, 2);
+                    continue;
+                }
+                const std::vector<int> cb = cut(0, cuse, ns, weight);
+                const int parts = (int)cb.size() - 1;
+                const int base = next_split;
+                for (int s2 = 0; s2 < parts; ++s2)
+                    push(L.fwd, b, r0, r1, 3, cb[s2], cb[s2 + 1], base, split_code(s2, parts));
+                next_split += parts * w;
+            }
+        }
+        lo = hi;
+    }
+    // The backward pass reuses the split rows: each combine resets its rows
+    // before its kernel ends.
+    const int fwd_split_rows = next_split;
+    next_split = 0;
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
+    for (size_t lo = 0; lo < order.size();) {
+        size_t hi = lo;
+        while (hi < order.size() && P.level_bwd[order[hi]] == P.level_bwd[order[lo]]) ++hi;
+        set_level_split(order, lo, hi);
+        for (size_t q = lo; q < hi; ++q) {
+            const int b = order[q], m = start[b + 1] - start[b];
+            const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
+            const bool big = m >= opt.split_gather;
+            const int wmax = big ? opt.bwd_cols_big : opt.bwd_cols;
+            for (int c0 = 0; c0 < m; c0 += wmax) {
+                const int c1 = std::min(m, c0 + wmax), w = c1 - c0;
+                // Row rho feeds the task columns c <= rho.
+                auto weight = [&](int rho) -> int64_t { return std::min(rho + 1, c1) - c0; };
+                int64_t entries = 0;
+                for (int rho = c0; rho < rows; ++rho) entries += weight(rho);
+                const int ns = big ? nsplits(entries) : 1;
+                if (ns == 1) {
+                    push(L.bwd, b, c0, c1, 0);
+                    continue;
+                }
+                const std::vector<int> hb = cut(c0, rows, ns, weight);
+                const int parts = (int)hb.size() - 1;
+                const int base = next_split;
+                for (int s2 = 0; s2 < parts; ++s2)
+                    push(L.bwd, b, c0, c1, 1, hb[s2], hb[s2 + 1], base, split_code(s2, parts));
+                next_split += parts * w;
+            }
+        }
+        lo = hi;
+    }
+    L.nsplit_rows = std::max(fwd_split_rows, next_split);
+
+    // 6. Leaf blocks' tasks first (forward) and last (backward), otherwise in
+    //    the same order.
+    auto is_leaf = [&](int b) {
+        const int m = start[b + 1] - start[b], R = P.rptr[b + 1] - P.rptr[b];
+        return opt.leaf_partition && P.level_fwd[b] == 0 && m <= kLeafMaxM && m + R <= kLeafMaxRows &&
+               m < opt.split_gather && opt.fwd_rows <= 64 && opt.bwd_cols <= 32;
+    };
+    auto partition = [&](std::vector<int> &tasks, bool leaf_first, int &count) {
+        std::vector<int> leaf, rest;
+        const size_t nt = tasks.size() / kTaskInts;
+        for (size_t t = 0; t < nt; ++t) {
+            std::vector<int> &dst = is_leaf(tasks[t * kTaskInts]) ? leaf : rest;
+            dst.insert(dst.end(), tasks.begin() + t * kTaskInts, tasks.begin() + (t + 1) * kTaskInts);
+        }
+        count = (int)(leaf.size() / kTaskInts);
+        tasks = leaf_first ? leaf : rest;
+        tasks.insert(tasks.end(), leaf_first ? rest.begin() : leaf.begin(), leaf_first ? rest.end() : leaf.end());
+    };
+    partition(L.fwd, true, L.n_leaf_fwd);
+    partition(L.bwd, false, L.n_leaf_bwd);
+    return L;
+}
+
+
+}  // namespace
+
 BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx, const BlockPlanOptions &opt) {
     BlockPlan P;
     P.n = n;
@@ -118,133 +295,16 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
         P.level_bwd[b] = lv;
     }
 
-    // 5. Tasks in level order (stable in block index).
-    int next_split = 0;   // split rows: their own array, numbered per plan
-    auto push = [](std::vector<int> &v, int b, int f, int l, int kind, int lo = 0, int hi = 0, int sb = 0,
-                   int ns = 0) {
-        const int rec[kTaskInts] = {b, f, l, kind, lo, hi, sb, ns};
-        v.insert(v.end(), rec, rec + kTaskInts);
-    };
-    // Cuts [lo, hi) into at most `parts` ranges of about equal weight.
-    auto cut = [](int lo, int hi, int parts, const std::function<int64_t(int)> &w) {
-        int64_t total = 0;
-        for (int x = lo; x < hi; ++x) total += w(x);
-        std::vector<int> b{lo};
-        int64_t acc = 0;
-        for (int x = lo; x < hi; ++x) {
-            acc += w(x);
-            if ((int)b.size() < parts && acc * parts >= total * (int64_t)b.size() && x + 1 < hi) b.push_back(x + 1);
-        }
-        b.push_back(hi);
-        return b;
-    };
-    auto nsplits = [&](int64_t entries) {
-        if (opt.split_entries <= 0 || entries <= opt.split_entries) return 1;
-        return (int)std::min<int64_t>(opt.max_split, (entries + opt.split_entries - 1) / opt.split_entries);
-    };
-    std::vector<int> order(nb);
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
-    for (size_t lo = 0; lo < order.size();) {
-        // One level at a time: the gather tasks of its large blocks first,
-        // then the product tasks, so every task's inputs come from tasks with
-        // smaller tickets.
-        size_t hi = lo;
-        while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
-        for (size_t q = lo; q < hi; ++q) {
-            const int b = order[q], m = start[b + 1] - start[b];
-            if (m < opt.split_gather) continue;
-            for (int r0 = 0; r0 < m; r0 += opt.fwd_rows) push(P.fwd_task, b, r0, std::min(m, r0 + opt.fwd_rows), 1);
-        }
-        for (size_t q = lo; q < hi; ++q) {
-            const int b = order[q], m = start[b + 1] - start[b];
-            const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
-            if (m < opt.split_gather) {
-                for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows)
-                    push(P.fwd_task, b, r0, std::min(rows, r0 + opt.fwd_rows), 0);
-                continue;
-            }
-            for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows_big) {
-                const int r1 = std::min(rows, r0 + opt.fwd_rows_big), w = r1 - r0;
-                // Column c feeds the task rows rho >= c (rho < m) and all rows >= m.
-                auto weight = [&](int c) -> int64_t { return r1 - std::max(r0, std::min(c, m)); };
-                const int cuse = std::min(r1, m);
-                int64_t entries = 0;
-                for (int c = 0; c < cuse; ++c) entries += weight(c);
-                const int ns = nsplits(entries);
-                if (ns == 1) {
-                    push(P.fwd_task, b, r0, r1, 2);
-                    continue;
-                }
-                const std::vector<int> cb = cut(0, cuse, ns, weight);
-                const int parts = (int)cb.size() - 1;
-                const int base = next_split;
-                for (int s2 = 0; s2 < parts; ++s2)
-                    push(P.fwd_task, b, r0, r1, 3, cb[s2], cb[s2 + 1], base, split_code(s2, parts));
-                next_split += parts * w;
-            }
-        }
-        lo = hi;
-    }
-    // The backward pass reuses the split rows: each combine resets its rows
-    // before its kernel ends.
-    const int fwd_split_rows = next_split;
-    next_split = 0;
-    std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
-    for (size_t lo = 0; lo < order.size();) {
-        size_t hi = lo;
-        while (hi < order.size() && P.level_bwd[order[hi]] == P.level_bwd[order[lo]]) ++hi;
-        for (size_t q = lo; q < hi; ++q) {
-            const int b = order[q], m = start[b + 1] - start[b];
-            const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
-            const bool big = m >= opt.split_gather;
-            const int wmax = big ? opt.bwd_cols_big : opt.bwd_cols;
-            for (int c0 = 0; c0 < m; c0 += wmax) {
-                const int c1 = std::min(m, c0 + wmax), w = c1 - c0;
-                // Row rho feeds the task columns c <= rho.
-                auto weight = [&](int rho) -> int64_t { return std::min(rho + 1, c1) - c0; };
-                int64_t entries = 0;
-                for (int rho = c0; rho < rows; ++rho) entries += weight(rho);
-                const int ns = big ? nsplits(entries) : 1;
-                if (ns == 1) {
-                    push(P.bwd_task, b, c0, c1, 0);
-                    continue;
-                }
-                const std::vector<int> hb = cut(c0, rows, ns, weight);
-                const int parts = (int)hb.size() - 1;
-                const int base = next_split;
-                for (int s2 = 0; s2 < parts; ++s2)
-                    push(P.bwd_task, b, c0, c1, 1, hb[s2], hb[s2 + 1], base, split_code(s2, parts));
-                next_split += parts * w;
-            }
-        }
-        lo = hi;
-    }
-    P.nsplit_rows = std::max(fwd_split_rows, next_split);
-
-    // 6. Leaf blocks' tasks first (forward) and last (backward), otherwise in
-    //    the same order.
-    auto is_leaf = [&](int b) {
-        const int m = start[b + 1] - start[b], R = P.rptr[b + 1] - P.rptr[b];
-        return P.level_fwd[b] == 0 && m <= kLeafMaxM && m + R <= kLeafMaxRows && m < opt.split_gather &&
-               opt.fwd_rows <= 64 && opt.bwd_cols <= 32;
-    };
-    auto partition = [&](std::vector<int> &tasks, bool leaf_first, int &count) {
-        std::vector<int> leaf, rest;
-        const size_t nt = tasks.size() / kTaskInts;
-        for (size_t t = 0; t < nt; ++t) {
-            std::vector<int> &dst = is_leaf(tasks[t * kTaskInts]) ? leaf : rest;
-            dst.insert(dst.end(), tasks.begin() + t * kTaskInts, tasks.begin() + (t + 1) * kTaskInts);
-        }
-        count = (int)(leaf.size() / kTaskInts);
-        tasks = leaf_first ? leaf : rest;
-        tasks.insert(tasks.end(), leaf_first ? rest.begin() : leaf.begin(), leaf_first ? rest.end() : leaf.end());
-    };
-    partition(P.fwd_task, true, P.n_leaf_fwd);
-    partition(P.bwd_task, false, P.n_leaf_bwd);
+    // 5. Task lists: one for narrow right-hand sides, one for wide ones.
+    TaskList narrow = make_tasks(P, narrow_shape(opt));
+    TaskList wide = make_tasks(P, wide_shape(opt));
+    P.fwd_task.swap(narrow.fwd);
+    P.bwd_task.swap(narrow.bwd);
+    P.n_leaf_fwd = narrow.n_leaf_fwd;
+    P.n_leaf_bwd = narrow.n_leaf_bwd;
+    P.fwd_task_wide.swap(wide.fwd);
+    P.bwd_task_wide.swap(wide.bwd);
+    P.nsplit_rows = std::max(narrow.nsplit_rows, wide.nsplit_rows);
     return P;
 }
 

@@ -49,6 +49,18 @@ struct BlockPlanOptions {
     // whose partial results a combine task adds (0 = no splitting).
     int split_entries = 4096;
     int max_split = 16;
+    // Levels whose large blocks hold fewer than level_tasks * split_entries
+    // entries split them finer, down to min_split_entries per task, so the
+    // level still has about level_tasks tasks (0 = fixed split_entries).
+    int level_tasks = 0;
+    int min_split_entries = 512;
+    // The task list for wide right-hand sides (the MFMA panel kernels, whose
+    // tasks carry k times more work per entry): narrower gathers and small
+    // blocks, more blocks with gather tasks, level-adaptive splits.
+    int wide_fwd_rows = 16;
+    int wide_split_gather = 33;
+    int wide_level_tasks = 384;
+    int wide_min_split_entries = 512;
     // Host threads for the dense block inverses (0 = hardware count, at most
     // 16). Batches already build one plan per thread and pass 1.
     int threads = 0;
@@ -88,7 +100,10 @@ struct BlockPlan {
     // last n_leaf_bwd backward tasks (the order stays valid for one launch
     // over all tasks).
     int n_leaf_fwd = 0, n_leaf_bwd = 0;
-    int nsplit_rows = 0;                   // rows of the split-result array
+    // The same two passes as tasks for wide right-hand sides (same records,
+    // BlockPlanOptions' wide_* shape, no leaf partition).
+    std::vector<int> fwd_task_wide, bwd_task_wide;
+    int nsplit_rows = 0;                   // rows of the split-result array (either list)
     int max_m = 0, max_rows = 0;           // largest m and m + |R|
     int64_t nnz_l = 0;
 };

@@ -68,6 +68,7 @@ struct DeviceBlockPlan {
     int nblocks = 0, nslots = 0, nsplit_rows = 0, max_m = 0, max_rows = 0;
     int n_fwd_tasks = 0, n_bwd_tasks = 0;
     int n_leaf_fwd = 0, n_leaf_bwd = 0;   // leaf tasks: forward prefix, backward suffix
+    int n_fwd_tasks_wide = 0, n_bwd_tasks_wide = 0;   // task lists of the panel (wide k) kernels
     int levels_fwd = 0, levels_bwd = 0;
     int64_t m_entries = 0;
     int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
@@ -75,6 +76,7 @@ struct DeviceBlockPlan {
     void *mfwd = nullptr, *mbwd = nullptr;
     int *cptr = nullptr, *cslot = nullptr;
     int *ftask = nullptr, *btask = nullptr;
+    int *ftask_wide = nullptr, *btask_wide = nullptr;
 
     void release() {
         dev_free(start);
@@ -87,6 +89,8 @@ struct DeviceBlockPlan {
         dev_free(cslot);
         dev_free(ftask);
         dev_free(btask);
+        dev_free(ftask_wide);
+        dev_free(btask_wide);
     }
 };
 
@@ -183,6 +187,13 @@ BlockPlanOptions plan_options() {
     if (const char *e = std::getenv("CHOLESPY_SPLIT_GATHER")) opt.split_gather = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_SPLIT_ENTRIES")) opt.split_entries = std::max(0, std::atoi(e));
     width("CHOLESPY_MAX_SPLIT", opt.max_split, 16);
+    if (const char *e = std::getenv("CHOLESPY_LEVEL_TASKS")) opt.level_tasks = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_MIN_SPLIT_ENTRIES")) opt.min_split_entries = std::max(1, std::atoi(e));
+    width("CHOLESPY_WIDE_FWD_ROWS", opt.wide_fwd_rows, 64);
+    if (const char *e = std::getenv("CHOLESPY_WIDE_SPLIT_GATHER")) opt.wide_split_gather = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_WIDE_LEVEL_TASKS")) opt.wide_level_tasks = std::max(0, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_WIDE_MIN_SPLIT_ENTRIES"))
+        opt.wide_min_split_entries = std::max(1, std::atoi(e));
     return opt;
 }
 
@@ -224,6 +235,8 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
     d.n_bwd_tasks = (int)(P.bwd_task.size() / kTaskInts);
     d.n_leaf_fwd = P.n_leaf_fwd;
     d.n_leaf_bwd = P.n_leaf_bwd;
+    d.n_fwd_tasks_wide = (int)(P.fwd_task_wide.size() / kTaskInts);
+    d.n_bwd_tasks_wide = (int)(P.bwd_task_wide.size() / kTaskInts);
     d.levels_fwd = d.levels_bwd = 0;
     for (int v : P.level_fwd) d.levels_fwd = std::max(d.levels_fwd, v + 1);
     for (int v : P.level_bwd) d.levels_bwd = std::max(d.levels_bwd, v + 1);
@@ -244,6 +257,10 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
         const std::vector<int> ft = device_tasks(P, P.fwd_task, blk_sys), bt = device_tasks(P, P.bwd_task, blk_sys);
         HIP_TRY(upload(&d.ftask, ft.data(), ft.size()));
         HIP_TRY(upload(&d.btask, bt.data(), bt.size()));
+        const std::vector<int> fw = device_tasks(P, P.fwd_task_wide, blk_sys),
+                               bw = device_tasks(P, P.bwd_task_wide, blk_sys);
+        HIP_TRY(upload(&d.ftask_wide, fw.data(), fw.size()));
+        HIP_TRY(upload(&d.btask_wide, bw.data(), bw.size()));
     }
     const size_t me = (size_t)d.m_entries;
     if (s->fbytes == 4) {
@@ -716,12 +733,18 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     for (int pass = 0; pass < 2; ++pass) {
         if (only_pass >= 0 && pass != only_pass) continue;
         const bool forward = pass == 0;
-        bf.ntasks = forward ? d.n_fwd_tasks : d.n_bwd_tasks;
-        bf.n_leaf = s->leaf_kernels ? (forward ? d.n_leaf_fwd : d.n_leaf_bwd) : 0;
-        bf.task = forward ? d.ftask : d.btask;
+        const int tile = block_tile(fbytes, bf.max_k);
+        if (is_panel_tile(tile)) {
+            bf.ntasks = forward ? d.n_fwd_tasks_wide : d.n_bwd_tasks_wide;
+            bf.n_leaf = 0;
+            bf.task = forward ? d.ftask_wide : d.btask_wide;
+        } else {
+            bf.ntasks = forward ? d.n_fwd_tasks : d.n_bwd_tasks;
+            bf.n_leaf = s->leaf_kernels ? (forward ? d.n_leaf_fwd : d.n_leaf_bwd) : 0;
+            bf.task = forward ? d.ftask : d.btask;
+        }
         bf.M = forward ? d.mfwd : d.mbwd;
         bf.trace = s->trace[forward ? 1 : 0];
-        const int tile = block_tile(fbytes, bf.max_k);
         int &grid = s->grid_blocks[forward ? 1 : 0][tile_slot(tile)];
         if (grid == 0) {
             const size_t lds = block_pass_lds(fbytes, forward ? d.max_m : d.max_rows, tile);
@@ -866,8 +889,9 @@ int cholespy_solver_set_trace(cholespy_gpu_solver *s, int enable) {
     dev_free(s->trace[0]);
     dev_free(s->trace[1]);
     if (!enable) return CHOLESPY_OK;
-    const size_t words[2] = {4 * (size_t)std::max(1, s->plan.n_bwd_tasks),
-                             4 * (size_t)std::max(1, s->plan.n_fwd_tasks)};
+    // Either task list (narrow or wide right-hand sides) may run while tracing.
+    const size_t words[2] = {4 * (size_t)std::max({1, s->plan.n_bwd_tasks, s->plan.n_bwd_tasks_wide}),
+                             4 * (size_t)std::max({1, s->plan.n_fwd_tasks, s->plan.n_fwd_tasks_wide})};
     for (int i = 0; i < 2; ++i) {
         HIP_TRY(dev_alloc(&s->trace[i], words[i]));
         HIP_TRY(hipMemset(s->trace[i], 0, words[i] * sizeof(unsigned long long)));
@@ -879,7 +903,8 @@ int cholespy_solver_get_trace(const cholespy_gpu_solver *s, int forward, uint64_
     if (!s || !out) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
     const unsigned long long *t = s->trace[forward ? 1 : 0];
     if (!t) return fail(CHOLESPY_ERR_STATE, "tracing is off");
-    const int64_t have = 4 * (int64_t)(forward ? s->plan.n_fwd_tasks : s->plan.n_bwd_tasks);
+    const int64_t have = 4 * (int64_t)(forward ? std::max(s->plan.n_fwd_tasks, s->plan.n_fwd_tasks_wide)
+                                               : std::max(s->plan.n_bwd_tasks, s->plan.n_bwd_tasks_wide));
     HIP_TRY(hipDeviceSynchronize());
     HIP_TRY(hipMemcpy(out, t, (size_t)std::min(have, words) * sizeof(uint64_t), hipMemcpyDeviceToHost));
     return CHOLESPY_OK;
@@ -971,7 +996,7 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     int B0 = 0, S0 = 0, C0 = 0, SP0 = 0;
     int64_t M0 = 0;
     struct TaskRef { int level, sys, idx; };
-    std::vector<TaskRef> ftasks, btasks;
+    std::vector<TaskRef> ftasks, btasks, ftasks_w, btasks_w;
     for (int i = 0; i < ns; ++i) {
         const BlockPlan &P = plans[i];
         const int R0 = s->sys_row0[i];
@@ -1004,6 +1029,10 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
                               i, (int)t});
         G.n_leaf_fwd += P.n_leaf_fwd;
         G.n_leaf_bwd += P.n_leaf_bwd;
+        for (size_t t = 0; t < P.fwd_task_wide.size() / kTaskInts; ++t)
+            ftasks_w.push_back({P.level_fwd[P.fwd_task_wide[kTaskInts * t]], i, (int)t});
+        for (size_t t = 0; t < P.bwd_task_wide.size() / kTaskInts; ++t)
+            btasks_w.push_back({P.level_bwd[P.bwd_task_wide[kTaskInts * t]], i, (int)t});
         G.max_m = std::max(G.max_m, P.max_m);
         G.max_rows = std::max(G.max_rows, P.max_rows);
         G.nnz_l += P.nnz_l;
@@ -1022,10 +1051,11 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     G.nsplit_rows = SP0;
     std::vector<int> blk0((size_t)ns + 1, 0);
     for (int i = 0; i < ns; ++i) blk0[i + 1] = blk0[i] + plans[i].nblocks;
-    auto emit = [&](std::vector<TaskRef> &refs, bool fwd, std::vector<int> &out) {
+    auto emit = [&](std::vector<TaskRef> &refs, bool fwd, bool wide, std::vector<int> &out) {
         std::stable_sort(refs.begin(), refs.end(), [](const TaskRef &x, const TaskRef &y) { return x.level < y.level; });
         for (const TaskRef &r : refs) {
-            const std::vector<int> &src = fwd ? plans[r.sys].fwd_task : plans[r.sys].bwd_task;
+            const BlockPlan &Q = plans[r.sys];
+            const std::vector<int> &src = wide ? (fwd ? Q.fwd_task_wide : Q.bwd_task_wide) : (fwd ? Q.fwd_task : Q.bwd_task);
             const int *t = &src[(size_t)kTaskInts * r.idx];
             // block and split-row base move by the system's offsets
             const int rec[kTaskInts] = {t[0] + blk0[r.sys], t[1], t[2], t[3], t[4], t[5],
@@ -1033,8 +1063,10 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
             out.insert(out.end(), rec, rec + kTaskInts);
         }
     };
-    emit(ftasks, true, G.fwd_task);
-    emit(btasks, false, G.bwd_task);
+    emit(ftasks, true, false, G.fwd_task);
+    emit(btasks, false, false, G.bwd_task);
+    emit(ftasks_w, true, true, G.fwd_task_wide);
+    emit(btasks_w, false, true, G.bwd_task_wide);
     plans.clear();
     HIP_TRY(hipMemcpy(s->perm, gperm.data(), (size_t)s->n * sizeof(int), hipMemcpyHostToDevice));
     int rc = upload_plan(s, G, &blk_sys);
@@ -1144,8 +1176,8 @@ int cholespy_solver_set_path(cholespy_gpu_solver *s, int path) {
 int cholespy_solver_plan_info(const cholespy_gpu_solver *s, int64_t *info, int count) {
     if (!s || !info) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
     const DeviceBlockPlan &d = s->plan;
-    const int64_t vals[] = {d.nblocks, d.m_entries, d.levels_fwd, d.levels_bwd,
-                            d.n_fwd_tasks, d.n_bwd_tasks, d.nslots, d.max_rows};
+    const int64_t vals[] = {d.nblocks,     d.m_entries,   d.levels_fwd, d.levels_bwd,       d.n_fwd_tasks,
+                            d.n_bwd_tasks, d.nslots,      d.max_rows,   d.n_fwd_tasks_wide, d.n_bwd_tasks_wide};
     for (int i = 0; i < count && i < (int)(sizeof(vals) / sizeof(vals[0])); ++i) info[i] = vals[i];
     return CHOLESPY_OK;
 }

@@ -60,6 +60,20 @@ __device__ __forceinline__ typename Bits<F>::U load_bits(const F *p) {
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// First read of a published entry, through the caches (a relaxed atomic at
+// workgroup scope is a plain load). Every published location holds the
+// sentinel or its final value and changes at most once per launch (launch
+// boundaries write back and invalidate the caches), so a stale cached copy
+// can only be the sentinel: callers pass it to ready_value, which then
+// waits at agent scope. Values read by many workgroups of one XCD (a large
+// block's r_B, a separator's z) then come from that XCD's L2.
+template <typename F>
+__device__ __forceinline__ typename Bits<F>::U peek_bits(const F *p) {
+    using U = typename Bits<F>::U;
+    return __hip_atomic_load(reinterpret_cast<const U *>(p), __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 __device__ __forceinline__ int load_int(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }

@@ -181,7 +181,7 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
                     for (int j = 0; j < kSlotGroup; ++j) {
                         const int idx = base + e * kTaskThreads;
                         const int sl = idx < total ? st.slot[ii[e]][g + j] : -1;
-                        us[e][j] = sl >= 0 ? load_bits(&pv[(long long)sl * k + tt[e]]) : U(0);
+                        us[e][j] = sl >= 0 ? peek_bits(&pv[(long long)sl * k + tt[e]]) : U(0);
                     }
 #pragma unroll
                 for (int e = 0; e < kStageElems; ++e)
@@ -199,7 +199,7 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
                 const int i = ii[e], t = tt[e];
                 for (int j = kStageSlots; j < st.cnt[i]; ++j) {   // rows with very many contributions
                     const F *p = &pv[(long long)a.cslot[st.first[i] + j] * k + t];
-                    v[e] += ready_value(p, load_bits(p), a.status);
+                    v[e] += ready_value(p, peek_bits(p), a.status);
                 }
                 if (to_rbuf)
                     publish(&rb[(long long)(a0 + q0 + i) * k + t], v[e]);
@@ -343,7 +343,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
                         for (int j = 0; j < kMaxTiles; ++j) {
                             const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
                             bu[q][j] = (j < w.ntiles && bok && t < k)
-                                           ? load_bits(rb0 + (long long)(col - clo) * k + 16 * j * w.tstride) : U(0);
+                                           ? peek_bits(rb0 + (long long)(col - clo) * k + 16 * j * w.tstride) : U(0);
                         }
                     }
 #pragma unroll
@@ -484,7 +484,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                             const int t = toff + 16 * j * w.tstride;
                             const bool ok = j < w.ntiles && bp[q] && t < k;
                             bu[q][j] = !ok ? U(0)
-                                           : (pubr[q] ? load_bits(bp[q] + 16 * j * w.tstride)
+                                           : (pubr[q] ? peek_bits(bp[q] + 16 * j * w.tstride)
                                                       : as_bits(bp[q][16 * j * w.tstride]));
                         }
 #pragma unroll

@@ -333,7 +333,7 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_trace",
              [](Class &self, bool forward) {
                  const std::vector<int64_t> info = self.plan_info();
-                 const int64_t words = 4 * (forward ? info[4] : info[5]);
+                 const int64_t words = 4 * (forward ? std::max(info[4], info[8]) : std::max(info[5], info[9]));
                  py::array_t<uint64_t> out(words);
                  throw_on_error(cholespy_solver_get_trace(self.gpu(), forward ? 1 : 0, out.mutable_data(), words));
                  return out;
@@ -377,8 +377,8 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_plan_info",
              [](const Class &self) {
                  const std::vector<int64_t> v = self.plan_info();
-                 const char *names[] = {"blocks", "m_entries", "levels_fwd", "levels_bwd",
-                                        "fwd_tasks", "bwd_tasks", "slots", "max_rows"};
+                 const char *names[] = {"blocks", "m_entries", "levels_fwd", "levels_bwd", "fwd_tasks",
+                                        "bwd_tasks", "slots", "max_rows", "fwd_tasks_wide", "bwd_tasks_wide"};
                  py::dict d;
                  for (size_t i = 0; i < v.size(); ++i) d[names[i]] = v[i];
                  return d;
@@ -444,10 +444,10 @@ void declare_batch(py::module_ &m, const char *name) {
         .def("_check", [](const Class &self) { throw_on_error(cholespy_solver_check(self.gpu())); })
         .def("_plan_info",
              [](const Class &self) {
-                 std::vector<int64_t> v(8, 0);
+                 std::vector<int64_t> v(10, 0);
                  throw_on_error(cholespy_solver_plan_info(self.gpu(), v.data(), (int)v.size()));
-                 const char *names[] = {"blocks", "m_entries", "levels_fwd", "levels_bwd",
-                                        "fwd_tasks", "bwd_tasks", "slots", "max_rows"};
+                 const char *names[] = {"blocks", "m_entries", "levels_fwd", "levels_bwd", "fwd_tasks",
+                                        "bwd_tasks", "slots", "max_rows", "fwd_tasks_wide", "bwd_tasks_wide"};
                  py::dict d;
                  for (size_t i = 0; i < v.size(); ++i) d[names[i]] = v[i];
                  return d;
@@ -457,9 +457,9 @@ void declare_batch(py::module_ &m, const char *name) {
              py::arg("on"))
         .def("_trace",
              [](Class &self, bool forward) {
-                 std::vector<int64_t> info(8, 0);
+                 std::vector<int64_t> info(10, 0);
                  throw_on_error(cholespy_solver_plan_info(self.gpu(), info.data(), (int)info.size()));
-                 const int64_t words = 4 * (forward ? info[4] : info[5]);
+                 const int64_t words = 4 * (forward ? std::max(info[4], info[8]) : std::max(info[5], info[9]));
                  py::array_t<uint64_t> out(words);
                  throw_on_error(cholespy_solver_get_trace(self.gpu(), forward ? 1 : 0, out.mutable_data(), words));
                  return out;

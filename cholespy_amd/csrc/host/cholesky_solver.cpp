@@ -134,7 +134,7 @@ void CholeskySolver<Float>::set_path(int path) {
 template <typename Float>
 std::vector<int64_t> CholeskySolver<Float>::plan_info() const {
     if (!gpu_) throw std::invalid_argument("Block plans only exist for GPU solvers.");
-    std::vector<int64_t> info(8, 0);
+    std::vector<int64_t> info(10, 0);
     throw_on_error(cholespy_solver_plan_info(gpu_, info.data(), (int)info.size()));
     return info;
 }

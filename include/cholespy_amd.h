@@ -183,7 +183,8 @@ int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);
 
 /* Block-plan statistics: info[0..count) = {blocks, dense entries of all
  * M_B, forward block levels, backward block levels, forward tasks, backward
- * tasks, partial slots, largest m + |R|}. No counterpart in the reference. */
+ * tasks, partial slots, largest m + |R|, forward tasks and backward tasks of
+ * the wide right-hand-side list}. No counterpart in the reference. */
 int cholespy_solver_plan_info(const cholespy_gpu_solver *solver, int64_t *info, int count);
 
 /* Copies of the block plan (diagnostics; no counterpart in the reference):
