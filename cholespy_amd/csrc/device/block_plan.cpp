@@ -297,7 +297,7 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
 
     // 5. Task lists: one for narrow right-hand sides, one for wide ones.
     TaskList narrow = make_tasks(P, narrow_shape(opt));
-    TaskList wide = make_tasks(P, wide_shape(opt));
+    TaskList wide = opt.wide_list ? make_tasks(P, wide_shape(opt)) : narrow;
     P.fwd_task.swap(narrow.fwd);
     P.bwd_task.swap(narrow.bwd);
     P.n_leaf_fwd = narrow.n_leaf_fwd;

@@ -61,6 +61,9 @@ struct BlockPlanOptions {
     int wide_split_gather = 33;
     int wide_level_tasks = 384;
     int wide_min_split_entries = 512;
+    // false: the wide list is a copy of the narrow one (batches, whose many
+    // systems already give each level enough tasks).
+    bool wide_list = true;
     // Host threads for the dense block inverses (0 = hardware count, at most
     // 16). Batches already build one plan per thread and pass 1.
     int threads = 0;

@@ -971,6 +971,7 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     BlockPlanOptions opt = plan_options();
     opt.threads = 1;
     if (ns >= 8 && !std::getenv("CHOLESPY_SPLIT_ENTRIES")) opt.split_entries = 0;
+    if (ns >= 8) opt.wide_list = false;
     std::vector<BlockPlan> plans((size_t)ns);
     {
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -1033,6 +1034,7 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
             ftasks_w.push_back({P.level_fwd[P.fwd_task_wide[kTaskInts * t]], i, (int)t});
         for (size_t t = 0; t < P.bwd_task_wide.size() / kTaskInts; ++t)
             btasks_w.push_back({P.level_bwd[P.bwd_task_wide[kTaskInts * t]], i, (int)t});
+
         G.max_m = std::max(G.max_m, P.max_m);
         G.max_rows = std::max(G.max_rows, P.max_rows);
         G.nnz_l += P.nnz_l;
@@ -1063,10 +1065,20 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
             out.insert(out.end(), rec, rec + kTaskInts);
         }
     };
+    // Batches of 8 or more systems plan without wide lists (each system's
+    // copy is its narrow list): a wide list splits a system's top levels for
+    // that system's own parallelism, which such a batch already has
+    // (measured on the 256-system batch). The merged wide list then follows
+    // the narrow order.
     emit(ftasks, true, false, G.fwd_task);
     emit(btasks, false, false, G.bwd_task);
-    emit(ftasks_w, true, true, G.fwd_task_wide);
-    emit(btasks_w, false, true, G.bwd_task_wide);
+    if (ns >= 8) {
+        G.fwd_task_wide = G.fwd_task;
+        G.bwd_task_wide = G.bwd_task;
+    } else {
+        emit(ftasks_w, true, true, G.fwd_task_wide);
+        emit(btasks_w, false, true, G.bwd_task_wide);
+    }
     plans.clear();
     HIP_TRY(hipMemcpy(s->perm, gperm.data(), (size_t)s->n * sizeof(int), hipMemcpyHostToDevice));
     int rc = upload_plan(s, G, &blk_sys);
