@@ -512,6 +512,31 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     return CHOLESPY_OK;
 }
 
+int cholespy_normalize_input(int n, int64_t nnz, const void *ii, const void *jj, int index_bytes, const void *x,
+                             int value_bytes, int type, int *colptr, int *rowidx, double *val, int64_t *entry_dst,
+                             int64_t *nnz_out) {
+    if (n < 0 || nnz < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "Sparse matrix: n_rows must be non-negative.");
+    if (nnz > 0x7fffffff) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "more than 2^31 - 1 input entries");
+    if (type < 0 || type > 2 || (index_bytes != 4 && index_bytes != 8) || (value_bytes != 4 && value_bytes != 8))
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "bad matrix type or element size");
+    if (!colptr || !nnz_out || (nnz > 0 && (!jj || !x || !rowidx || !val)) || (type != 2 && !ii) ||
+        (type == 2 && nnz > 0 && !ii))
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = cholespy_init();
+    if (rc != CHOLESPY_OK) return rc;
+    NormalizedInput out;
+    HIP_TRY(normalize_on_device(n, nnz, ii, jj, index_bytes, x, value_bytes, type, out));
+    if (!out.invalid.empty()) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, out.invalid);
+    std::memcpy(colptr, out.colptr.data(), ((size_t)n + 1) * sizeof(int));
+    if (!out.rowidx.empty()) {
+        std::memcpy(rowidx, out.rowidx.data(), out.rowidx.size() * sizeof(int));
+        std::memcpy(val, out.val.data(), out.val.size() * sizeof(double));
+    }
+    if (entry_dst && nnz > 0) std::memcpy(entry_dst, out.entry_dst.data(), (size_t)nnz * sizeof(int64_t));
+    *nnz_out = (int64_t)out.rowidx.size();
+    return CHOLESPY_OK;
+}
+
 int cholespy_solver_refactor(cholespy_gpu_solver *s, const double *val, int64_t nnz) {
     if (!s || !val) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
     if (s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "refactor is not available for batches");

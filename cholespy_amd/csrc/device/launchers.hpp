@@ -1,5 +1,8 @@
 // Host-side launchers for the gfx950 kernels (one definition per .hip file).
 #pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -21,6 +24,21 @@ constexpr size_t kQueueWords = 16 * 16 + 16;   // counters + done word (padded)
 constexpr int kRowBlock = 256;
 // Rows per ticket of the row kernels (consecutive in dependency order).
 constexpr int kRowChunk = 4;
+
+// Device-side input normalisation (normalize.hip): COO / CSR / CSC arrays in
+// device memory (type as MatrixType: 0 CSC, 1 CSR, 2 COO; indices of
+// index_bytes = 4 or 8, values of value_bytes = 4 or 8) -> the lower CSC
+// on the host, bit-identical to the host normalisation, and for each input
+// entry the output position it is summed into (-1: above the diagonal).
+// An invalid input leaves `invalid` non-empty (the host path's message).
+struct NormalizedInput {
+    std::vector<int> colptr, rowidx;
+    std::vector<double> val;
+    std::vector<int64_t> entry_dst;
+    std::string invalid;
+};
+hipError_t normalize_on_device(int n, int64_t nnz, const void *ii, const void *jj, int index_bytes, const void *x,
+                               int value_bytes, int type, NormalizedInput &out);
 
 // Largest co-resident grid (workgroups) for a persistent kernel.
 int max_coresident_blocks(const void *kernel, int block, size_t lds_bytes);

@@ -186,6 +186,18 @@ std::vector<T> to_host(const DLView &v) {
     return out;
 }
 
+// Whether the constructor's GPU arrays can go to cholespy_normalize_input
+// as they are: one-dimensional and contiguous, indices int32 or int64 (the
+// same for both), values float32 or float64.
+bool device_normalizable(const DLView &ii, const DLView &jj, const DLView &x) {
+    auto contiguous = [](const DLView &v) { return v.ndim() == 1 && (v.shape(0) <= 1 || v.stride(0) == 1); };
+    const dl::DataType &ti = ii.dtype(), &tj = jj.dtype(), &tx = x.dtype();
+    const bool idx_ok = ti.code == dl::kInt && ti.lanes == 1 && (ti.bits == 32 || ti.bits == 64) &&
+                        tj.code == ti.code && tj.bits == ti.bits && tj.lanes == 1;
+    const bool val_ok = tx.code == dl::kFloat && tx.lanes == 1 && (tx.bits == 32 || tx.bits == 64);
+    return idx_ok && val_ok && contiguous(ii) && contiguous(jj) && contiguous(x);
+}
+
 template <typename Float>
 void declare_solver(py::module_ &m, const char *name, const char *doc) {
     using Class = CholeskySolver<Float>;
@@ -223,6 +235,27 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
                  if (dev != Dev::CPU && dev != Dev::GPU)
                      throw std::invalid_argument("Unsupported input device! Only CPU and CUDA arrays are supported.");
                  if (dev == Dev::GPU) throw_on_error(cholespy_init());   // src/main.cpp:51
+                 // GPU arrays that are contiguous, with 32- or 64-bit integer
+                 // indices and float values, are normalised on the device
+                 // (only the lower CSC comes to the host); others take the
+                 // reference's route of copying the raw arrays first.
+                 if (dev == Dev::GPU && device_normalizable(ii, jj, x)) {
+                     const int64_t nnz = x.shape(0);
+                     LowerCSC A;
+                     A.n = (int)n_rows;
+                     A.colptr.resize((size_t)n_rows + 1);
+                     A.rowidx.resize((size_t)nnz);
+                     A.val.resize((size_t)nnz);
+                     std::vector<int64_t> edst((size_t)nnz);
+                     int64_t m = 0;
+                     py::gil_scoped_release release;
+                     throw_on_error(cholespy_normalize_input((int)n_rows, nnz, ii.data(), jj.data(), ii.dtype().bits / 8,
+                                                             x.data(), x.dtype().bits / 8, (int)type, A.colptr.data(),
+                                                             A.rowidx.data(), A.val.data(), edst.data(), &m));
+                     A.rowidx.resize((size_t)m);
+                     A.val.resize((size_t)m);
+                     return std::make_unique<Class>(std::move(A), std::move(edst));
+                 }
                  std::vector<int> iv = to_host<int>(ii), jv = to_host<int>(jj);
                  std::vector<double> xv = to_host<double>(x);
                  py::gil_scoped_release release;

@@ -21,6 +21,17 @@ CholeskySolver<Float>::CholeskySolver(int n_rows, int64_t nnz, const int *ii, co
                                       const double *x, MatrixType type, bool cpu)
     : n_(n_rows), cpu_(cpu) {
     A_ = normalize_input(n_rows, nnz, ii, jj, x, type, &entry_dst_);
+    setup();
+}
+
+template <typename Float>
+CholeskySolver<Float>::CholeskySolver(LowerCSC A, std::vector<int64_t> entry_dst)
+    : n_(A.n), cpu_(false), A_(std::move(A)), entry_dst_(std::move(entry_dst)) {
+    setup();
+}
+
+template <typename Float>
+void CholeskySolver<Float>::setup() {
     symbolic_ = symbolic_factor(A_);
     factor_ = numeric_factor(symbolic_, A_.val);
     if (cpu_) return;

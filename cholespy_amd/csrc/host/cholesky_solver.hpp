@@ -20,6 +20,9 @@ public:
     // src/cholesky_solver.h:36 / src/cholesky_solver.cpp:151-198
     CholeskySolver(int n_rows, int64_t nnz, const int *ii, const int *jj, const double *x,
                    MatrixType type, bool cpu);
+    // GPU mode from an input already normalised on the device
+    // (cholespy_normalize_input): A's lower CSC and the input entry map.
+    CholeskySolver(LowerCSC A, std::vector<int64_t> entry_dst);
     ~CholeskySolver();
     CholeskySolver(const CholeskySolver &) = delete;
     CholeskySolver &operator=(const CholeskySolver &) = delete;
@@ -79,6 +82,7 @@ private:
     // The level analysis of L and L' (src/cholesky_solver.cpp:299-376) is
     // only needed by the row schedule and by levels(): it runs on first use.
     void analyse() const;
+    void setup();   // host factor and, in GPU mode, the device upload
     mutable bool analysed_ = false;
     mutable int n_levels_lower_ = 0, n_levels_upper_ = 0;
 };

@@ -108,6 +108,23 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *solver, const int *perm,
  * (src/cholesky_solver.cpp:151-198, 299-376). */
 int cholespy_solver_refactor(cholespy_gpu_solver *solver, const double *val, int64_t nnz);
 
+/* Device-side input normalisation. ii, jj, x are device pointers in the
+ * layout of `type` (0 = CSC: ii column pointer; 1 = CSR: ii row pointer;
+ * 2 = COO: ii rows, jj columns), indices of index_bytes = 4 or 8, values of
+ * value_bytes = 4 (float) or 8 (double). Writes to host memory the lower
+ * triangle (row >= col) in compressed columns, rows ascending and
+ * duplicates summed in input order: colptr[n + 1], rowidx and val (room for
+ * nnz entries; *nnz_out are written), and, when entry_dst is not NULL,
+ * entry_dst[nnz] = the output position each input entry is summed into, or
+ * -1 above the diagonal. The result is bit-identical to the reference's host
+ * path, which copies the arrays to the host first (src/main.cpp:55-67) and
+ * sorts and sums there (src/cholesky_solver.cpp:8-149; CHOLMOD stype = -1 at
+ * 211-220). Invalid indices or pointers return CHOLESPY_ERR_INVALID_ARGUMENT
+ * with the host path's message. Synchronous. */
+int cholespy_normalize_input(int n, int64_t nnz, const void *ii, const void *jj, int index_bytes,
+                             const void *x, int value_bytes, int type, int *colptr, int *rowidx,
+                             double *val, int64_t *entry_dst, int64_t *nnz_out);
+
 /* Level-set analysis of L (lower = 1) or L' (lower = 0) on the GPU followed
  * by the stable counting sort of rows by level. Replaces analyze_cuda()
  * (src/cholesky_solver.cpp:299-376) and the kernels analysis_lower /
