@@ -146,7 +146,7 @@ struct cholespy_gpu_solver {
     unsigned int *qctr = nullptr;
     QueueState queues[6];
     unsigned long long *trace[2] = {nullptr, nullptr};   // block pass timelines [backward, forward]
-    std::vector<int> plan_copy[6];        // host copies of the plan for cholespy_solver_plan_arrays
+    std::vector<int> plan_copy[8];        // host copies of the plan for cholespy_solver_plan_arrays
     // Batches of independent systems (cholespy_batch_*): rows are
     // concatenated; per system its size, partial-slot count and host factor
     // until finalize; per call the RHS counts and vector bases.
@@ -247,6 +247,8 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
     s->plan_copy[3] = P.level_bwd;
     s->plan_copy[4] = P.fwd_task;
     s->plan_copy[5] = P.bwd_task;
+    s->plan_copy[6] = P.fwd_task_wide;
+    s->plan_copy[7] = P.bwd_task_wide;
     HIP_TRY(upload(&d.start, P.start.data(), P.start.size()));
     HIP_TRY(upload(&d.rptr, P.rptr.data(), P.rptr.size()));
     HIP_TRY(upload(&d.rlist, P.rlist.data(), P.rlist.size()));
@@ -1221,7 +1223,7 @@ int cholespy_solver_plan_info(const cholespy_gpu_solver *s, int64_t *info, int c
 
 int cholespy_solver_plan_arrays(const cholespy_gpu_solver *s, int which, int32_t *out, int64_t count,
                                 int64_t *size) {
-    if (!s || which < 0 || which > 5 || (count > 0 && !out)) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "bad argument");
+    if (!s || which < 0 || which > 7 || (count > 0 && !out)) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "bad argument");
     const std::vector<int> &v = s->plan_copy[which];
     if (size) *size = (int64_t)v.size();
     const int64_t nc = std::min<int64_t>(count, (int64_t)v.size());

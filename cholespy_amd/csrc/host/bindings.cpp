@@ -120,9 +120,10 @@ py::object as_exportable(py::handle obj) {
 
 // Plan arrays of a GPU handle (cholespy_solver_plan_arrays) as a dict of numpy arrays.
 py::dict plan_arrays(const cholespy_gpu_solver *h) {
-    const char *names[] = {"start", "rptr", "level_fwd", "level_bwd", "fwd_task", "bwd_task"};
+    const char *names[] = {"start", "rptr", "level_fwd", "level_bwd", "fwd_task", "bwd_task", "fwd_task_wide",
+                           "bwd_task_wide"};
     py::dict d;
-    for (int w = 0; w < 6; ++w) {
+    for (int w = 0; w < 8; ++w) {
         int64_t size = 0;
         throw_on_error(cholespy_solver_plan_arrays(h, w, nullptr, 0, &size));
         py::array_t<int32_t> a(size);

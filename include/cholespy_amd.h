@@ -208,7 +208,8 @@ int cholespy_solver_plan_info(const cholespy_gpu_solver *solver, int64_t *info, 
  * which = 0 block row starts (blocks + 1), 1 partial-slot ranges
  * (blocks + 1), 2 forward block levels (blocks), 3 backward block levels
  * (blocks), 4 forward tasks and 5 backward tasks (CHOLESPY_TASK_INTS ints
- * each: block, first, last, kind, lo, hi, split-row base, splits). Copies
+ * each: block, first, last, kind, lo, hi, split-row base, splits), 6 and 7
+ * the forward and backward tasks of the wide right-hand-side list. Copies
  * min(count, size) ints to out; *size (if not null) receives the full size. */
 #define CHOLESPY_TASK_INTS 8
 int cholespy_solver_plan_arrays(const cholespy_gpu_solver *solver, int which, int32_t *out, int64_t count,
