@@ -23,6 +23,7 @@ struct BlockPassArgs {
     const F *M;             // forward: column-major packed; backward: row-major packed
     const int *cptr;        // contribution lists (forward)
     const int *cslot;
+    const int *gslot;       // each row's first kGatherSlots slots, fixed stride, -1 padded
     const int *perm;
     const F *b;
     F *y, *z, *part, *rbuf;
@@ -179,9 +180,10 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
     // One pair per thread in FP64 keeps the gather's registers within the
     // product loop's budget (occupancy), two in FP32.
     constexpr int kPairs = sizeof(F) == 4 ? 2 : 1;
+    static_assert(kSlotBatch == kGatherSlots, "the first slot batch is a row of gslot");
     const int total = (r1 - r0) * width;
     for (int base = threadIdx.x; base < total; base += kTaskThreads * kPairs) {
-        int p0[kPairs], p1[kPairs], col[kPairs];
+        int p0[kPairs], p1[kPairs], col[kPairs], grow[kPairs];
         F v[kPairs];
 #pragma unroll
         for (int q = 0; q < kPairs; ++q) {
@@ -189,9 +191,11 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
             const int i = idx / width, t = idx - i * width;
             p0[q] = p1[q] = 0;
             col[q] = t0 + t;
+            grow[q] = -1;
             v[q] = F(0);
             if (idx < total && t < kt) {
                 const int row = a0 + r0 + i;
+                grow[q] = row;
                 v[q] = bv[(long long)a.perm[row] * k + col[q]];
                 p0[q] = a.cptr[row];
                 p1[q] = a.cptr[row + 1];
@@ -201,13 +205,29 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
             bool any = false;
             int sl[kPairs][kSlotBatch];
 #pragma unroll
-            for (int q = 0; q < kPairs; ++q)
+            for (int q = 0; q < kPairs; ++q) {
+                if (r == 0) {
+                    // The row's first slots from the fixed-stride table: no
+                    // wait for cptr (which is only needed past kSlotBatch).
+                    const int4 *g = reinterpret_cast<const int4 *>(a.gslot + (size_t)max(grow[q], 0) * kGatherSlots);
 #pragma unroll
-                for (int j = 0; j < kSlotBatch; ++j) {
-                    const int p = p0[q] + r + j;
-                    sl[q][j] = p < p1[q] ? a.cslot[p] : -1;
-                    any |= sl[q][j] >= 0;
+                    for (int h = 0; h < kSlotBatch / 4; ++h) {
+                        const int4 w = grow[q] >= 0 ? g[h] : make_int4(-1, -1, -1, -1);
+                        sl[q][4 * h] = w.x;
+                        sl[q][4 * h + 1] = w.y;
+                        sl[q][4 * h + 2] = w.z;
+                        sl[q][4 * h + 3] = w.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < kSlotBatch; ++j) {
+                        const int p = p0[q] + r + j;
+                        sl[q][j] = p < p1[q] ? a.cslot[p] : -1;
+                    }
                 }
+#pragma unroll
+                for (int j = 0; j < kSlotBatch; ++j) any |= sl[q][j] >= 0;
+            }
             if (!any) break;
 #pragma unroll
             for (int h = 0; h < kSlotBatch; h += kValBatch) {

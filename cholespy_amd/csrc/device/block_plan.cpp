@@ -273,6 +273,10 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
         std::vector<int> next(P.contrib_ptr.begin(), P.contrib_ptr.end() - 1);
         for (int s = 0; s < nslots; ++s) P.contrib_slot[next[P.rlist[s]]++] = s;
     }
+    P.gslot.assign((size_t)n * kGatherSlots, -1);
+    for (int i = 0; i < n; ++i)
+        for (int p = P.contrib_ptr[i], j = 0; p < P.contrib_ptr[i + 1] && j < kGatherSlots; ++p, ++j)
+            P.gslot[(size_t)i * kGatherSlots + j] = P.contrib_slot[p];
     std::vector<int> slot_block(nslots);
     for (int b = 0; b < nb; ++b)
         for (int s = P.rptr[b]; s < P.rptr[b + 1]; ++s) slot_block[s] = b;

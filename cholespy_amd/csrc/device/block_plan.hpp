@@ -20,6 +20,7 @@
 namespace cholespy_amd {
 
 constexpr int kTaskInts = 8;   // ints per task record (CHOLESPY_TASK_INTS of the C ABI)
+constexpr int kGatherSlots = 16;   // contribution slots per row in BlockPlan::gslot
 // Leaf blocks: blocks without child blocks (forward level 0) of at most
 // kLeafMaxM rows and kLeafMaxRows rows of M_B. They take nothing from other
 // blocks in the forward pass and give nothing to other blocks in the
@@ -78,6 +79,10 @@ struct BlockPlan {
     // contrib_ptr / contrib_slot: for each row i, the slots whose partial sums
     // belong to row i (CSR over rows).
     std::vector<int> contrib_ptr, contrib_slot;
+    // The first kGatherSlots entries of each row's contribution list at a
+    // fixed stride (row i at gslot[i * kGatherSlots], -1 padded): a gather
+    // loads them without first loading contrib_ptr[i].
+    std::vector<int> gslot;
     // Values of M_B in two layouts (one per pass), offsets per block.
     //   forward : column-major packed  (column c holds rows c .. m+|R|-1)
     //   backward: row-major packed     (row rho holds columns 0 .. min(rho, m-1))

@@ -74,7 +74,7 @@ struct DeviceBlockPlan {
     int *start = nullptr, *rptr = nullptr, *rlist = nullptr;
     int64_t *moff = nullptr;
     void *mfwd = nullptr, *mbwd = nullptr;
-    int *cptr = nullptr, *cslot = nullptr;
+    int *cptr = nullptr, *cslot = nullptr, *gslot = nullptr;
     int *ftask = nullptr, *btask = nullptr;
     int *ftask_wide = nullptr, *btask_wide = nullptr;
 
@@ -87,6 +87,7 @@ struct DeviceBlockPlan {
         dev_free(mbwd);
         dev_free(cptr);
         dev_free(cslot);
+        dev_free(gslot);
         dev_free(ftask);
         dev_free(btask);
         dev_free(ftask_wide);
@@ -255,6 +256,7 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
     HIP_TRY(upload(&d.moff, P.moff.data(), P.moff.size()));
     HIP_TRY(upload(&d.cptr, P.contrib_ptr.data(), P.contrib_ptr.size()));
     HIP_TRY(upload(&d.cslot, P.contrib_slot.data(), P.contrib_slot.size()));
+    HIP_TRY(upload(&d.gslot, P.gslot.data(), P.gslot.size()));
     {
         const std::vector<int> ft = device_tasks(P, P.fwd_task, blk_sys), bt = device_tasks(P, P.bwd_task, blk_sys);
         HIP_TRY(upload(&d.ftask, ft.data(), ft.size()));
@@ -739,6 +741,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.moff = d.moff;
     bf.cptr = d.cptr;
     bf.cslot = d.cslot;
+    bf.gslot = d.gslot;
     bf.perm = s->perm;
     bf.b = b;
     bf.x = x;
@@ -1046,6 +1049,7 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
         G.mbwd.insert(G.mbwd.end(), P.mbwd.begin(), P.mbwd.end());
         for (int r = 0; r < s->sys_n[i]; ++r) G.contrib_ptr.push_back(P.contrib_ptr[r] + C0);
         for (int v : P.contrib_slot) G.contrib_slot.push_back(v + S0);
+        for (int v : P.gslot) G.gslot.push_back(v >= 0 ? v + S0 : -1);
         // Sort keys: the level, with leaf tasks first (forward) or last
         // (backward) as in each system's plan.
         const size_t nf = P.fwd_task.size() / kTaskInts, nb = P.bwd_task.size() / kTaskInts;

@@ -108,6 +108,7 @@ struct BlockPassBuffers {
     const int64_t *moff = nullptr;
     const void *M = nullptr;       // forward or backward layout of M_B
     const int *cptr = nullptr, *cslot = nullptr;
+    const int *gslot = nullptr;    // first kGatherSlots contribution slots per row (-1 padded)
     const int *perm = nullptr;
     const void *b = nullptr;
     void *y = nullptr, *z = nullptr, *part = nullptr, *x = nullptr;

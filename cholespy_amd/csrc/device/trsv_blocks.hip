@@ -570,6 +570,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.M = static_cast<const F *>(bf.M);
     a.cptr = bf.cptr;
     a.cslot = bf.cslot;
+    a.gslot = bf.gslot;
     a.perm = bf.perm;
     a.b = static_cast<const F *>(bf.b);
     a.y = static_cast<F *>(bf.y);

@@ -151,8 +151,18 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
             st.perm[tid] = a.perm[row];
             st.cnt[tid] = p1 - p0;
             st.first[tid] = p0;
+            // The row's first slots from the fixed-stride table (-1 padded),
+            // loaded together with cptr instead of after it.
+            static_assert(kStageSlots == kGatherSlots, "a staged row is a row of gslot");
+            const int4 *g = reinterpret_cast<const int4 *>(a.gslot + (size_t)row * kGatherSlots);
 #pragma unroll
-            for (int j = 0; j < kStageSlots; ++j) st.slot[tid][j] = p0 + j < p1 ? a.cslot[p0 + j] : -1;
+            for (int h = 0; h < kStageSlots / 4; ++h) {
+                const int4 w = g[h];
+                st.slot[tid][4 * h] = w.x;
+                st.slot[tid][4 * h + 1] = w.y;
+                st.slot[tid][4 * h + 2] = w.z;
+                st.slot[tid][4 * h + 3] = w.w;
+            }
         }
         __syncthreads();
         const int total = nr * k;
