@@ -28,7 +28,13 @@ namespace {
 
 struct TaskShape {
     int fwd_rows, bwd_cols, split_gather, fwd_rows_big, bwd_cols_big, split_entries, max_split;
+    // level_tasks > 0: a level that gives at least level_tasks tasks with the
+    // "busy" shape (busy_fwd_rows, busy_split_gather, bwd_cols_big) uses it;
+    // a level with fewer uses the finer fwd_rows / split_gather /
+    // fine_bwd_cols_big and splits its large blocks down to
+    // min_split_entries entries a task.
     int level_tasks, min_split_entries;
+    int busy_fwd_rows, busy_split_gather, fine_bwd_cols_big;
     bool leaf_partition;
 };
 
@@ -38,12 +44,16 @@ struct TaskList {
 };
 
 TaskShape narrow_shape(const BlockPlanOptions &o) {
-    return TaskShape{o.fwd_rows, o.bwd_cols, o.split_gather, o.fwd_rows_big, o.bwd_cols_big,
-                     o.split_entries, o.max_split, o.level_tasks, o.min_split_entries, true};
+    return TaskShape{o.fwd_rows,          o.bwd_cols,          o.split_gather, o.fwd_rows_big,
+                     o.bwd_cols_big,      o.split_entries,     o.max_split,    o.level_tasks,
+                     o.min_split_entries, o.fwd_rows,          o.split_gather, o.bwd_cols_big,
+                     true};
 }
 TaskShape wide_shape(const BlockPlanOptions &o) {
-    return TaskShape{o.wide_fwd_rows, o.bwd_cols, o.wide_split_gather, o.fwd_rows_big, o.bwd_cols_big,
-                     o.split_entries, o.max_split, o.wide_level_tasks, o.wide_min_split_entries, false};
+    return TaskShape{o.wide_fwd_rows,          o.bwd_cols,      o.wide_split_gather, o.fwd_rows_big,
+                     o.bwd_cols_big,           o.split_entries, o.max_split,         o.wide_level_tasks,
+                     o.wide_min_split_entries, o.fwd_rows,      o.split_gather,      o.wide_fine_bwd_cols_big,
+                     false};
 }
 
 // Tasks of both passes in level order (stable in block index) for one task shape.
@@ -78,14 +88,14 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
         if (level_split <= 0 || entries <= level_split) return 1;
         return (int)std::min<int64_t>(opt.max_split, (entries + level_split - 1) / level_split);
     };
-    auto set_level_split = [&](const std::vector<int> &order, size_t lo, size_t hi) {
+    auto set_level_split = [&](const std::vector<int> &order, size_t lo, size_t hi, int sg) {
         level_split = opt.split_entries;
         if (opt.level_tasks <= 0 || opt.split_entries <= 0) return;
         int64_t entries = 0;
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q];
             const int64_t m = start[b + 1] - start[b], R = P.rptr[b + 1] - P.rptr[b];
-            if (m >= opt.split_gather) entries += m * (m + 1) / 2 + R * m;
+            if (m >= sg) entries += m * (m + 1) / 2 + R * m;
         }
         const int64_t want = entries / opt.level_tasks;
         level_split = (int)std::max<int64_t>(opt.min_split_entries, std::min<int64_t>(opt.split_entries, want));
@@ -100,18 +110,34 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
         // smaller tickets.
         size_t hi = lo;
         while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
-        set_level_split(order, lo, hi);
+        // The level's shape: the busy one when it already gives enough tasks.
+        int rows_l = opt.fwd_rows, sg_l = opt.split_gather;
+        if (opt.level_tasks > 0) {
+            int64_t natural = 0;
+            for (size_t q = lo; q < hi; ++q) {
+                const int b = order[q], m = start[b + 1] - start[b];
+                const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
+                natural += m < opt.busy_split_gather
+                               ? (rows + opt.busy_fwd_rows - 1) / opt.busy_fwd_rows
+                               : (m + opt.busy_fwd_rows - 1) / opt.busy_fwd_rows + (rows + opt.fwd_rows_big - 1) / opt.fwd_rows_big;
+            }
+            if (natural >= opt.level_tasks) {
+                rows_l = opt.busy_fwd_rows;
+                sg_l = opt.busy_split_gather;
+            }
+        }
+        set_level_split(order, lo, hi, sg_l);
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q], m = start[b + 1] - start[b];
-            if (m < opt.split_gather) continue;
-            for (int r0 = 0; r0 < m; r0 += opt.fwd_rows) push(L.fwd, b, r0, std::min(m, r0 + opt.fwd_rows), 1);
+            if (m < sg_l) continue;
+            for (int r0 = 0; r0 < m; r0 += rows_l) push(L.fwd, b, r0, std::min(m, r0 + rows_l), 1);
         }
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q], m = start[b + 1] - start[b];
             const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
-            if (m < opt.split_gather) {
-                for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows)
-                    push(L.fwd, b, r0, std::min(rows, r0 + opt.fwd_rows), 0);
+            if (m < sg_l) {
+                for (int r0 = 0; r0 < rows; r0 += rows_l)
+                    push(L.fwd, b, r0, std::min(rows, r0 + rows_l), 0);
                 continue;
             }
             for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows_big) {
@@ -146,12 +172,23 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
     for (size_t lo = 0; lo < order.size();) {
         size_t hi = lo;
         while (hi < order.size() && P.level_bwd[order[hi]] == P.level_bwd[order[lo]]) ++hi;
-        set_level_split(order, lo, hi);
+        int sg_l = opt.split_gather, bcols_big_l = opt.bwd_cols_big;
+        if (opt.level_tasks > 0) {
+            int64_t natural = 0;
+            for (size_t q = lo; q < hi; ++q) {
+                const int m = start[order[q] + 1] - start[order[q]];
+                const int w = m >= opt.busy_split_gather ? opt.bwd_cols_big : opt.bwd_cols;
+                natural += (m + w - 1) / w;
+            }
+            if (natural >= opt.level_tasks) sg_l = opt.busy_split_gather;
+            else bcols_big_l = opt.fine_bwd_cols_big;
+        }
+        set_level_split(order, lo, hi, sg_l);
         for (size_t q = lo; q < hi; ++q) {
             const int b = order[q], m = start[b + 1] - start[b];
             const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
-            const bool big = m >= opt.split_gather;
-            const int wmax = big ? opt.bwd_cols_big : opt.bwd_cols;
+            const bool big = m >= sg_l;
+            const int wmax = big ? bcols_big_l : opt.bwd_cols;
             for (int c0 = 0; c0 < m; c0 += wmax) {
                 const int c1 = std::min(m, c0 + wmax), w = c1 - c0;
                 // Row rho feeds the task columns c <= rho.

@@ -56,12 +56,15 @@ struct BlockPlanOptions {
     int level_tasks = 0;
     int min_split_entries = 512;
     // The task list for wide right-hand sides (the MFMA panel kernels, whose
-    // tasks carry k times more work per entry): narrower gathers and small
-    // blocks, more blocks with gather tasks, level-adaptive splits.
+    // tasks carry k times more work per entry): on levels that would give
+    // fewer than wide_level_tasks tasks with the narrow shape, narrower
+    // gathers and small blocks, more blocks with gather tasks, narrower
+    // backward tasks and finer splits; other levels keep the narrow shape.
     int wide_fwd_rows = 16;
     int wide_split_gather = 33;
     int wide_level_tasks = 384;
     int wide_min_split_entries = 512;
+    int wide_fine_bwd_cols_big = 16;   // columns per backward task of large blocks on sparse levels
     // false: the wide list is a copy of the narrow one (batches, whose many
     // systems already give each level enough tasks).
     bool wide_list = true;
