@@ -62,7 +62,7 @@ struct BlockPlanOptions {
     // backward tasks and finer splits; other levels keep the narrow shape.
     int wide_fwd_rows = 16;
     int wide_split_gather = 33;
-    int wide_level_tasks = 384;
+    int wide_level_tasks = 512;
     int wide_min_split_entries = 512;
     int wide_fine_bwd_cols_big = 16;   // columns per backward task of large blocks on sparse levels
     // false: the wide list is a copy of the narrow one (batches, whose many
