@@ -107,6 +107,7 @@ def test_batch_with_split_reductions(monkeypatch):
     """A batch whose plans contain split and combine tasks (small split size)
     matches the oracle per system, with per-system right-hand-side counts."""
     monkeypatch.setenv("CHOLESPY_SPLIT_ENTRIES", "512")
+    monkeypatch.setenv("CHOLESPY_SPLIT_GATHER", "65")
     systems = workloads.batch_systems(count=10, seed=8, n_min=500, n_max=4000)
     batch = BatchedSolverD([(n, idx[0], idx[1], v, MatrixType.COO) for n, v, idx, k in systems])
     assert (batch._plan_arrays()["fwd_task"][:, 3] == 3).any()
