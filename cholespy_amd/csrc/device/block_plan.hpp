@@ -101,8 +101,8 @@ struct BlockPlan {
     // 3 = split s of nsplit (split = split_code(s, nsplit)): as 2 over
     //     columns [lo, hi) only, result published to the split rows
     //     sbase + s * (last - first) + (rho - first) (an array of its own,
-    //     k values a row); the split that finishes last (a counter per
-    //     group, indexed by sbase) adds all nsplit results in split order
+    //     k values a row); split nsplit - 1 (the group's last ticket)
+    //     waits for the others, adds all nsplit results in split order
     //     and finishes the rows as kind 2 would.
     // Backward kinds: 0 = columns [first, last) of M_B; 1 = split s of
     // nsplit over rows [lo, hi), combined the same way and finished as 0.

@@ -125,7 +125,6 @@ struct cholespy_gpu_solver {
     void *part = nullptr;                 // block path partial slots (nslots x k)
     void *spart = nullptr;                // block path split results (nsplit_rows x k)
     size_t spart_elems = 0;
-    int *scnt = nullptr;                  // split-group counters (nsplit_rows ints, zero at rest)
     void *rbuf = nullptr;                 // block path gathered right-hand sides (n x k)
     int k_cur = 0;                        // layout the work buffers hold
     int path = CHOLESPY_PATH_BLOCKS;
@@ -228,9 +227,6 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
     d.nblocks = P.nblocks;
     d.nslots = P.rptr[P.nblocks];
     d.nsplit_rows = P.nsplit_rows;
-    dev_free(s->scnt);
-    HIP_TRY(dev_alloc(&s->scnt, (size_t)std::max(1, P.nsplit_rows)));
-    HIP_TRY(hipMemset(s->scnt, 0, (size_t)std::max(1, P.nsplit_rows) * sizeof(int)));
     d.max_m = P.max_m;
     d.max_rows = P.max_rows;
     d.n_fwd_tasks = (int)(P.fwd_task.size() / kTaskInts);
@@ -423,7 +419,6 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->z);
     dev_free(s->part);
     dev_free(s->spart);
-    dev_free(s->scnt);
     dev_free(s->rbuf);
     dev_free(s->arowptr);
     dev_free(s->acols);
@@ -641,7 +636,6 @@ int cholespy_solver_check(cholespy_gpu_solver *s) {
                 HIP_TRY(launch_fill_sentinel(s->part, s->part_elems, s->fbytes, nullptr));
             if (s->spart && s->spart_elems)
                 HIP_TRY(launch_fill_sentinel(s->spart, s->spart_elems, s->fbytes, nullptr));
-            if (s->scnt) HIP_TRY(hipMemset(s->scnt, 0, (size_t)std::max(1, s->plan.nsplit_rows) * sizeof(int)));
             if (s->rbuf) HIP_TRY(launch_fill_sentinel(s->rbuf, s->work_elems, s->fbytes, nullptr));
             HIP_TRY(hipDeviceSynchronize());
         }
@@ -750,7 +744,6 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.z = s->z;
     bf.part = s->part;
     bf.spart = s->spart;
-    bf.scnt = s->scnt;
     bf.rbuf = s->rbuf;
     bf.status = s->status;
     bf.xmode = xmode;

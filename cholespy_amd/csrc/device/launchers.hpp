@@ -113,7 +113,6 @@ struct BlockPassBuffers {
     const void *b = nullptr;
     void *y = nullptr, *z = nullptr, *part = nullptr, *x = nullptr;
     void *spart = nullptr;         // split results of split tasks (split rows x k, sentinel-protected)
-    int *scnt = nullptr;           // per split group: finished splits (zero between launches)
     void *rbuf = nullptr;          // gathered r_B of large blocks (n x k, sentinel-protected)
     int *status = nullptr;
     // Backward epilogue: 0 = x (solver precision) = z; 1 = x (double) = z;

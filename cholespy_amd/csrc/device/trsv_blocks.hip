@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
     // One chunk of r_B (a.chunk rows x KT); after the products the same
     // space holds the 256 x KT per-thread sums.
     F *rv = reinterpret_cast<F *>(smem);
-    __shared__ int s_next, s_flag;
+    __shared__ int s_next;
     const int tid = threadIdx.x;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
             }
             __syncthreads();
         }
-        if (kind == 3 && last_split(a, task.sbase, nsp, &s_flag))
+        if (kind == 3 && combines_splits(sidx, nsp))
             combine_splits(a, sp, last - first, k, task.sbase, nsp,
                            [&](int i, int t, F v) { finish(first + i, t, v); });
         if (a.trace && tid == 0) {
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
     // One chunk of v = [y_B ; z_R] (a.chunk rows x KT); after the products
     // the same space holds the 256 x KT per-thread sums.
     F *vs = reinterpret_cast<F *>(smem);
-    __shared__ int s_next, s_flag;
+    __shared__ int s_next;
     const int tid = threadIdx.x;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -344,7 +344,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
             }
             __syncthreads();
         }
-        if (kind == 1 && last_split(a, task.sbase, nsp, &s_flag))
+        if (kind == 1 && combines_splits(sidx, nsp))
             combine_splits(a, sp, last - first, k, task.sbase, nsp,
                            [&](int i, int t, F v) { finish(first + i, t, v); });
         // Restore the sentinel in this block's partial slots for the next solve
@@ -578,7 +578,6 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.part = static_cast<F *>(bf.part);
     a.rbuf = static_cast<F *>(bf.rbuf);
     a.spart = static_cast<F *>(bf.spart);
-    a.scnt = bf.scnt;
     a.x = bf.x;
     a.status = bf.status;
     a.xmode = bf.xmode;

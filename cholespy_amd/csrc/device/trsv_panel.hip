@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
     constexpr int kSteps = kStepsFor<NT>();
     extern __shared__ __align__(16) unsigned char smem[];
     F *rv = reinterpret_cast<F *>(smem);
-    __shared__ int s_next, s_flag;
+    __shared__ int s_next;
     __shared__ GatherStage stage;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int qid = blockIdx.x % a.qa.nq;
@@ -390,7 +390,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
                     }
                 }
             }
-            if (kind == 3 && last_split(a, task.sbase, nsp, &s_flag))
+            if (kind == 3 && combines_splits(sidx, nsp))
                 combine_splits(a, sp, last - first, k, task.sbase, nsp,
                                [&](int i, int t, F v) { finish(first + i, t, v); });
         }
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
     constexpr int kSteps = kStepsFor<NT>();
     extern __shared__ __align__(16) unsigned char smem[];
     F *vs = reinterpret_cast<F *>(smem);
-    __shared__ int s_next, s_flag;
+    __shared__ int s_next;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int qid = blockIdx.x % a.qa.nq;
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassAr
                     }
                 }
             }
-            if (kind == 1 && last_split(a, task.sbase, nsp, &s_flag))
+            if (kind == 1 && combines_splits(sidx, nsp))
                 combine_splits(a, sp, last - first, k, task.sbase, nsp,
                                [&](int i, int t, F v) { finish(first + i, t, v); });
         }
