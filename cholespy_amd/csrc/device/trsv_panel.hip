@@ -57,6 +57,18 @@ template <> struct Mfma<double> {
 // few accumulators and loads more steps ahead: kSteps(NT) MFMA steps have
 // their A (and B) loads issued together.
 template <int NT> constexpr int kStepsFor() { return NT <= 2 ? 8 : 4; }
+// Waves per SIMD the compiler must allow (amdgpu_waves_per_eu), chosen per
+// kernel as the most that compile with no register spills on gfx950. By
+// default the compiler keeps more accumulators in AGPRs and gets 2-3 waves;
+// more waves put more tasks in flight where many small tasks run at once
+// (the leaf levels, batches). FP32, 8 tiles: 224 -> 166 registers forward
+// (3 waves), 116 backward (4). FP32, up to 4 tiles: 4 waves forward, 5
+// backward. FP64 forward would spill and keeps the default; FP64 backward
+// with up to 4 tiles fits 4 waves.
+template <typename F, int NT, bool Forward> constexpr int kMinWavesFor() {
+    if (sizeof(F) == 4) return NT >= 8 ? 3 : 4;
+    return !Forward && NT <= 4 ? 4 : 1;
+}
 // Backward tasks stage the R-row indices they read in LDS (up to this many).
 constexpr int kStageRlist = 4096;
 
@@ -243,7 +255,8 @@ __device__ __forceinline__ void reduce_splits(F *red, const WaveWork &w, int wav
 }
 
 template <typename F, int NT>
-__global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArgs<F> a) {
+__global__ __launch_bounds__(kTaskThreads) __attribute__((amdgpu_waves_per_eu(kMinWavesFor<F, NT, true>())))
+void block_forward_panel(BlockPassArgs<F> a) {
     using Acc = typename Mfma<F>::Acc;
     constexpr int kMaxTiles = NT;
     constexpr int kSteps = kStepsFor<NT>();
@@ -407,7 +420,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward_panel(BlockPassArg
 }
 
 template <typename F, int NT>
-__global__ __launch_bounds__(kTaskThreads) void block_backward_panel(BlockPassArgs<F> a) {
+__global__ __launch_bounds__(kTaskThreads) __attribute__((amdgpu_waves_per_eu(kMinWavesFor<F, NT, false>())))
+void block_backward_panel(BlockPassArgs<F> a) {
     using Acc = typename Mfma<F>::Acc;
     constexpr int kMaxTiles = NT;
     constexpr int kSteps = kStepsFor<NT>();
