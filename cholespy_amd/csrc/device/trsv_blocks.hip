@@ -530,13 +530,14 @@ __global__ __launch_bounds__(kTaskThreads) void leaf_backward(BlockPassArgs<F> a
             }
         }
     }
-    // Restore the sentinel in the block's partial slots and r_B rows for the
-    // next solve (once per block: its first column task).
+    // Restore the sentinel in the block's partial slots for the next solve
+    // (once per block: its first column task). Its r_B rows in rbuf need no
+    // reset: a leaf block has m < split_gather, so no forward task of the
+    // narrow list (the only one these launches serve) gathers it into rbuf,
+    // and a wide solve's backward pass resets what its forward pass wrote.
     if (first == 0) {
         const F sent = from_bits<F>(Bits<F>::sentinel);
         for (int idx = lane; idx < R * k; idx += kWave) pv[(long long)r0 * k + idx] = sent;
-        F *rb = a.rbuf + vm.vbase;
-        for (int idx = lane; idx < m * k; idx += kWave) rb[(long long)a0 * k + idx] = sent;
     }
     if (a.trace && lane == 0) {
         a.trace[4 * (size_t)tk + 2] = wall_clock64();
