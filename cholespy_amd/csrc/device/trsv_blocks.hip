@@ -41,7 +41,7 @@ constexpr int kBLanes = 32;     // backward: most columns of M_B per task
 #endif
 constexpr int kMU = CHOLESPY_KMU;   // M loads issued together per thread
 #ifndef CHOLESPY_KPRE
-#define CHOLESPY_KPRE 8
+#define CHOLESPY_KPRE 16
 #endif
 // M entries per thread loaded before the task's right-hand-side gather, which
 // may wait on other tasks: their latency overlaps the wait and the gather.
