@@ -29,13 +29,14 @@ namespace {
 struct TaskShape {
     int fwd_rows, bwd_cols, split_gather, fwd_rows_big, bwd_cols_big, split_entries, max_split;
     // level_tasks > 0: a level that gives at least level_tasks tasks with the
-    // "busy" shape (busy_fwd_rows, busy_split_gather, bwd_cols_big) uses it;
-    // a level with fewer uses the finer fwd_rows / split_gather /
-    // fine_bwd_cols_big and splits its large blocks down to
-    // min_split_entries entries a task.
+    // "busy" shape (busy_fwd_rows, busy_split_gather, busy_fwd_rows_big,
+    // bwd_cols_big) uses it; a level with fewer uses the finer fwd_rows /
+    // split_gather / fwd_rows_big / fine_bwd_cols_big and splits its large
+    // blocks down to min_split_entries entries a task.
     int level_tasks, min_split_entries;
     int busy_fwd_rows, busy_split_gather, fine_bwd_cols_big;
     bool leaf_partition;
+    int busy_fwd_rows_big;
 };
 
 struct TaskList {
@@ -47,13 +48,13 @@ TaskShape narrow_shape(const BlockPlanOptions &o) {
     return TaskShape{o.fwd_rows,          o.bwd_cols,          o.split_gather, o.fwd_rows_big,
                      o.bwd_cols_big,      o.split_entries,     o.max_split,    o.level_tasks,
                      o.min_split_entries, o.fwd_rows,          o.split_gather, o.bwd_cols_big,
-                     true};
+                     true,                o.fwd_rows_big};
 }
 TaskShape wide_shape(const BlockPlanOptions &o) {
-    return TaskShape{o.wide_fwd_rows,          o.bwd_cols,      o.wide_split_gather, o.fwd_rows_big,
+    return TaskShape{o.wide_fwd_rows,          o.bwd_cols,      o.wide_split_gather, o.wide_fwd_rows_big,
                      o.bwd_cols_big,           o.split_entries, o.max_split,         o.wide_level_tasks,
                      o.wide_min_split_entries, o.fwd_rows,      o.split_gather,      o.wide_fine_bwd_cols_big,
-                     false};
+                     false,                    o.fwd_rows_big};
 }
 
 // Tasks of both passes in level order (stable in block index) for one task shape.
@@ -111,7 +112,7 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
         size_t hi = lo;
         while (hi < order.size() && P.level_fwd[order[hi]] == P.level_fwd[order[lo]]) ++hi;
         // The level's shape: the busy one when it already gives enough tasks.
-        int rows_l = opt.fwd_rows, sg_l = opt.split_gather;
+        int rows_l = opt.fwd_rows, sg_l = opt.split_gather, rows_big_l = opt.fwd_rows_big;
         if (opt.level_tasks > 0) {
             int64_t natural = 0;
             for (size_t q = lo; q < hi; ++q) {
@@ -119,11 +120,13 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
                 const int rows = m + (P.rptr[b + 1] - P.rptr[b]);
                 natural += m < opt.busy_split_gather
                                ? (rows + opt.busy_fwd_rows - 1) / opt.busy_fwd_rows
-                               : (m + opt.busy_fwd_rows - 1) / opt.busy_fwd_rows + (rows + opt.fwd_rows_big - 1) / opt.fwd_rows_big;
+                               : (m + opt.busy_fwd_rows - 1) / opt.busy_fwd_rows +
+                                     (rows + opt.busy_fwd_rows_big - 1) / opt.busy_fwd_rows_big;
             }
             if (natural >= opt.level_tasks) {
                 rows_l = opt.busy_fwd_rows;
                 sg_l = opt.busy_split_gather;
+                rows_big_l = opt.busy_fwd_rows_big;
             }
         }
         set_level_split(order, lo, hi, sg_l);
@@ -140,8 +143,8 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
                     push(L.fwd, b, r0, std::min(rows, r0 + rows_l), 0);
                 continue;
             }
-            for (int r0 = 0; r0 < rows; r0 += opt.fwd_rows_big) {
-                const int r1 = std::min(rows, r0 + opt.fwd_rows_big), w = r1 - r0;
+            for (int r0 = 0; r0 < rows; r0 += rows_big_l) {
+                const int r1 = std::min(rows, r0 + rows_big_l), w = r1 - r0;
                 // Column c feeds the task rows rho >= c (rho < m) and all rows >= m.
                 auto weight = [&](int c) -> int64_t { return r1 - std::max(r0, std::min(c, m)); };
                 const int cuse = std::min(r1, m);

@@ -63,6 +63,10 @@ struct BlockPlanOptions {
     // gathers and small blocks, more blocks with gather tasks, narrower
     // backward tasks and finer splits; other levels keep the narrow shape.
     int wide_fwd_rows = 16;
+    // Rows per forward product task of large blocks on the wide list's
+    // sparse levels: 16 (one 16-row MFMA group, the tile columns spread over
+    // the 4 wavefronts); busy levels keep fwd_rows_big.
+    int wide_fwd_rows_big = 16;
     int wide_split_gather = 33;
     int wide_level_tasks = 512;
     int wide_min_split_entries = 512;

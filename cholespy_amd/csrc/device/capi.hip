@@ -190,6 +190,7 @@ BlockPlanOptions plan_options() {
     if (const char *e = std::getenv("CHOLESPY_LEVEL_TASKS")) opt.level_tasks = std::max(0, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_MIN_SPLIT_ENTRIES")) opt.min_split_entries = std::max(1, std::atoi(e));
     width("CHOLESPY_WIDE_FWD_ROWS", opt.wide_fwd_rows, 64);
+    width("CHOLESPY_WIDE_FWD_ROWS_BIG", opt.wide_fwd_rows_big, 64);
     if (const char *e = std::getenv("CHOLESPY_WIDE_SPLIT_GATHER")) opt.wide_split_gather = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_WIDE_LEVEL_TASKS")) opt.wide_level_tasks = std::max(0, std::atoi(e));
     width("CHOLESPY_WIDE_FINE_BWD_COLS_BIG", opt.wide_fine_bwd_cols_big, 32);
