@@ -253,6 +253,15 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
 
     // 1. Partition into contiguous blocks: small whole subtrees, then chains
     //    (j and j + 1 share a block when parent(j) = j + 1), capped at max_block.
+    int max_block = opt.max_block;
+    if (max_block <= 0) {
+        int longest = n > 0 ? 1 : 0;
+        for (int j = 0, run = 1; j + 1 < n; ++j) {
+            run = parent[j] == j + 1 ? run + 1 : 1;
+            longest = std::max(longest, run);
+        }
+        max_block = longest >= 4096 ? 2048 : 1024;
+    }
     std::vector<int> &start = P.start;
     int j = 0;
     while (j < n) {
@@ -269,7 +278,7 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
         }
         start.push_back(j);
         int e = j;
-        while (e + 1 < n && parent[e] == e + 1 && e + 1 - j + 1 <= opt.max_block) ++e;
+        while (e + 1 < n && parent[e] == e + 1 && e + 1 - j + 1 <= max_block) ++e;
         j = e + 1;
     }
     const int nb = (int)start.size();
