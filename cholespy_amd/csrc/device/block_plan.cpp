@@ -260,7 +260,8 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
             run = parent[j] == j + 1 ? run + 1 : 1;
             longest = std::max(longest, run);
         }
-        max_block = longest >= 4096 ? 2048 : 1024;
+        max_block = 1024;
+        while (max_block < 4096 && 2 * max_block <= longest) max_block *= 2;
     }
     std::vector<int> &start = P.start;
     int j = 0;

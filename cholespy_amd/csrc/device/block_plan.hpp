@@ -35,10 +35,11 @@ constexpr int split_index(int code) { return code >> 8; }
 
 struct BlockPlanOptions {
     // Rows per block of a chain (the dense inverse is m x m). 0 = automatic:
-    // 2048 when the longest chain of the elimination tree has >= 4096 rows
-    // (large 3D separators: fewer block levels; the 64^3 Laplacian at k = 32
-    // solves 10 % faster), else 1024 (2048 costs icosphere(7) 5 %, whose
-    // top chains would become single ~2,000-row blocks).
+    // the largest power of two <= the elimination tree's longest chain,
+    // within [1024, 4096]. Large 3D separators then take fewer block levels
+    // (the 64^3 Laplacian's 4096-row chains: k = 32 refined 7.22 -> 6.29 ms
+    // with 4096 against 1024), while icosphere(7), whose longest chain has
+    // ~2,000 rows, keeps 1024 (2048 costs it 5 %).
     int max_block = 0;
     int leaf_max = 64;      // whole subtrees up to this size become one block
     int fwd_rows = 64;      // rows of M_B per forward task
