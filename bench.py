@@ -6,7 +6,9 @@ sides = vertex positions + N(0, 0.01) noise (seed 0), factored once on the
 host (nested dissection + supernodal multifrontal LL').
 
 Paths timed in the same process, inputs resident in HBM:
-  refined  FP32 block solve + 2 FP64 refinement steps
+  refined  FP32 block solve + FP64 refinement steps until the relative
+           residual is <= 1e-12 (the steps needed are reported; each step
+           reads the residual norms back to the host)
   f64      FP64 block solve
   f32      FP32 block solve alone (reported, below the accuracy bar)
   plain    FP64 row-per-wavefront level-scheduled solve (the reference's
@@ -31,7 +33,9 @@ over the pass's two kernels), read from the summary kept beside this file in
 profiles/ when it names the same kernels.
 cpu_baseline: the oracle's FP64 simplicial solve (oracle/oracle.c,
 oracle_host_solve: what cholmod_solve does for a simplicial LL' factor) on
-the same factor, one thread, bounded to ~10 s of work.
+the same factor, one thread, bounded to ~10 s of work; next to it the
+all-cores variant, the right-hand-side columns shared out over
+min(k, nproc) threads (each column is an independent solve).
 
 --profile runs only the headline path for `steps` solves and then a plain
 256 MiB device copy (the known-size kernel that scales the counters); it is
@@ -53,7 +57,8 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "RHS-cols solved/s at ≤1e-10 rel.err vs FP64 CHOLMOD; % HBM roofline; vs plain"
 HBM_PEAK_GBS = 8000.0
-REFINE_ITERS = 2
+REFINE_TOL = 1e-12     # relative residual the refined path iterates to
+REFINE_MAX_STEPS = 6
 TOL = 1e-10
 
 
@@ -162,8 +167,11 @@ def main():
 
     P = {"rows": cholespy_amd._core._PATH_ROWS, "blocks": cholespy_amd._core._PATH_BLOCKS}
 
+    refine_steps = {}
+
     def run_refined():
-        sf._solve_refined(b64, x64, REFINE_ITERS)
+        steps, res = sf._solve_refined_tol(b64, x64, REFINE_TOL, REFINE_MAX_STEPS)
+        refine_steps["steps"], refine_steps["residual"] = steps, res
 
     def run_f64():
         sd.solve(b64, x64)
@@ -343,10 +351,24 @@ def main():
             oracle.host_solve(fac, rhs)
             times.append(time.perf_counter() - c0)
         med = float(np.median(times))
+        nproc = os.cpu_count() or 1
+        threads = max(1, min(k, nproc))
+        ptimes = []
+        t_end = time.perf_counter() + args.cpu_seconds / 2
+        oracle.host_solve_parallel(fac, rhs, threads)
+        while time.perf_counter() < t_end or len(ptimes) < 5:
+            c0 = time.perf_counter()
+            oracle.host_solve_parallel(fac, rhs, threads)
+            ptimes.append(time.perf_counter() - c0)
+        pmed = float(np.median(ptimes))
         cpu = {"value": k / med, "unit": "RHS-cols/s", "cores": 1, "kind": "port",
                "sample": f"{len(times)} FP64 solves of the same system (k={k}) on the same factor, "
-                         f"median {med * 1000:.2f} ms, single thread"}
-        log(f"cpu baseline: {med * 1000:.2f} ms/solve ({len(times)} runs)")
+                         f"median {med * 1000:.2f} ms, single thread",
+               "all_cores": {"value": k / pmed, "unit": "RHS-cols/s", "cores": threads, "nproc": nproc,
+                             "sample": f"{len(ptimes)} solves, the {k} columns over {threads} threads "
+                                       f"(nproc {nproc}), median {pmed * 1000:.2f} ms"}}
+        log(f"cpu baseline: {med * 1000:.2f} ms/solve ({len(times)} runs); "
+            f"{threads} threads: {pmed * 1000:.2f} ms (nproc {nproc})")
 
     out = {
         "metric": METRIC,
@@ -365,7 +387,9 @@ def main():
             "workload": workload_name,
             "n": n, "nnz_A": nnz_a, "nnz_L": nnz_l, "k": k,
             "path": "blocks FP32 + FP64 refinement" if headline == "refined" else "blocks FP64",
-            "refine_iters": REFINE_ITERS if headline == "refined" else 0,
+            "refine_tol": REFINE_TOL,
+            "refine_steps": refine_steps.get("steps"),
+            "refine_residual": refine_steps.get("residual"),
             "rel_err": final_err,
             "solves_per_s": 1000.0 / ms_step,
             "ms": {name: r["ms"] for name, r in results.items()},
@@ -373,6 +397,7 @@ def main():
             "rel_errs": {name: r["rel_err"] for name, r in results.items()},
             "vs_plain": results["plain"]["ms"] / ms_step,
             "refined_vs_plain": results["plain"]["ms"] / results["refined"]["ms"],
+            "refined_vs_f64": results["f64"]["ms"] / results["refined"]["ms"],
             "solve_roofline_frac": solve_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "block_plan": plan,
             "reference_benchmark": ref_bench,
@@ -454,7 +479,13 @@ def bench_precision(args):
             steps = it
             break
     reps = max(5, args.steps // 20)
-    ms_refined = time_events(lambda: sf._solve_refined(b64, x64, steps or 6), reps)
+    # The timed path: refinement to the same bar, the stopping test on the
+    # device (the residual norms read back before each step).
+    tol_steps, tol_res = sf._solve_refined_tol(b64, x64, 1e-12, 6)
+    torch.cuda.synchronize()
+    xt = x64.cpu().numpy()
+    log(f"adaptive refinement: {tol_steps} steps, device residual {tol_res:.2e}, host residual {residual(xt):.2e}")
+    ms_refined = time_events(lambda: sf._solve_refined_tol(b64, x64, 1e-12, 6), reps)
     ms_f32 = time_events(lambda: sf.solve(b32, x32), reps)
     sf._check()
     del sf
@@ -477,6 +508,8 @@ def bench_precision(args):
             "n": n, "k": k, "nnz_L": int(len(fac["val"])),
             "kappa_estimate": (12.0 + 1e-4) / 1e-4,
             "refine_steps": steps, "refinement": trail,
+            "adaptive": {"steps": tol_steps, "device_residual": tol_res, "residual": residual(xt),
+                         "rel_err": rel_err(xt)},
             "ms": {"refined": ms_refined, "f32": ms_f32, "f64": ms_f64},
             "f64": {"residual": residual(x64h), "rel_err": rel_err(x64h)},
             "refined_vs_f64": ms_f64 / ms_refined,

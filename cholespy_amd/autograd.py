@@ -73,6 +73,12 @@ class SolveFunction(torch.autograd.Function):
             if rows is None or cols is None:
                 raise ValueError("gradients for the matrix entries need rows and cols (coo_indices)")
             dtype, device, shape = ctx.values_meta
+            n = x.shape[0]
+            if rows.numel() > 0:
+                lo = min(int(rows.min()), int(cols.min()))
+                hi = max(int(rows.max()), int(cols.max()))
+                if lo < 0 or hi >= n:
+                    raise IndexError(f"rows / cols hold an index outside [0, {n}): [{lo}, {hi}]")
             if x.is_cuda:
                 gv = torch.empty(shape, dtype=dtype, device=device)
                 _core._entry_gradient(rows, cols, lam, x, gv)

@@ -93,8 +93,8 @@ __device__ __forceinline__ Task load_task(const BlockPassArgs<F> &a, int tk) {
 // Value of a published entry: one agent-scope load, and the bounded wait of
 // common.hpp only when it still holds the sentinel.
 template <typename F>
-__device__ __forceinline__ F ready_value(const F *p, typename Bits<F>::U u, int *status) {
-    return u == Bits<F>::sentinel ? wait_value(p, status) : from_bits<F>(u);
+__device__ __forceinline__ F ready_value(const F *p, typename Bits<F>::U u, int *status, uint32_t limit) {
+    return u == Bits<F>::sentinel ? wait_value(p, status, limit) : from_bits<F>(u);
 }
 
 // Split epilogue: the split that adds its group's results is split
@@ -131,7 +131,7 @@ __device__ __forceinline__ void combine_splits(const BlockPassArgs<F> &a, F *sp,
         F v = F(0);
 #pragma unroll
         for (int s = 0; s < kMaxSplit; ++s)
-            if (s < nsplit) v += ready_value(p0 + s * stride, u[s], a.status);
+            if (s < nsplit) v += ready_value(p0 + s * stride, u[s], a.status, a.qa.spin_limit);
         const F sent = from_bits<F>(Bits<F>::sentinel);
 #pragma unroll
         for (int s = 0; s < kMaxSplit; ++s)
@@ -243,7 +243,7 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
                     for (int j = 0; j < kValBatch; ++j) {
                         F x = from_bits<F>(u[q][j]);
                         if (sl[q][h + j] >= 0 && u[q][j] == Bits<F>::sentinel)
-                            x = wait_value(&pv[(long long)sl[q][h + j] * k + col[q]], a.status);
+                            x = wait_value(&pv[(long long)sl[q][h + j] * k + col[q]], a.status, a.qa.spin_limit);
                         v[q] += x;
                     }
             }
@@ -293,7 +293,7 @@ __device__ __forceinline__ void gather_rows(const BlockPassArgs<F> &a, F *dst, i
             const int idx = base + j * kTaskThreads;
             if (idx >= total) break;
             F v = from_bits<F>(u[j]);
-            if (pub[j] && u[j] == Bits<F>::sentinel) v = wait_value(ad[j], a.status);
+            if (pub[j] && u[j] == Bits<F>::sentinel) v = wait_value(ad[j], a.status, a.qa.spin_limit);
             const int i = idx / width;
             dst[i * stride + (idx - i * width)] = v;
         }

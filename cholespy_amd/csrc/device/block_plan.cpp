@@ -309,6 +309,10 @@ BlockPlan build_block_plan_structure(int n, const int *colptr, const int *rowidx
         P.rlist.insert(P.rlist.end(), rows_b.begin(), rows_b.end());
         P.rptr[b + 1] = (int)P.rlist.size();
         const int64_t m = e - a, R = (int64_t)rows_b.size();
+        // The panel kernels index one block's M_B with 32-bit offsets, and
+        // their column-major offset c * (m + R) reaches m * (m + R).
+        if (m * (m + R) >= (int64_t(1) << 31))
+            throw std::length_error("block plan: a block's dense M_B would exceed 2^31 entries");
         P.moff[b + 1] = P.moff[b] + m * (m + 1) / 2 + R * m;
         P.max_m = std::max<int>(P.max_m, (int)m);
         P.max_rows = std::max<int>(P.max_rows, (int)(m + R));

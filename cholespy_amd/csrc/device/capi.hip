@@ -8,9 +8,11 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <stdexcept>
 #include <string>
 #include <thread>
 #include <vector>
@@ -120,7 +122,15 @@ struct cholespy_gpu_solver {
     void *uvals = nullptr;
     int *order[2] = {nullptr, nullptr};   // ticket -> row per pass
     int *lvl1 = nullptr;                  // analysis scratch: level + 1
-    int *status = nullptr;                // [0] error code, [1] abort flag
+    // Watchdog words [0] error code, [1] abort flag: host memory mapped into
+    // the device, so the host reads them with no device call. status is the
+    // device address the kernels get, hstatus the host one.
+    int *status = nullptr;
+    volatile int *hstatus = nullptr;
+    // A forward pass (cholespy_solver_solve_pass 0) whose backward pass has
+    // not run yet: its partial sums, gathered right-hand sides and y are
+    // still published, so the next forward pass must start from a refill.
+    bool fwd_pending = false;
     void *y = nullptr, *z = nullptr;      // per-solve work buffers (n x k)
     void *part = nullptr;                 // block path partial slots (nslots x k)
     void *spart = nullptr;                // block path split results (nsplit_rows x k)
@@ -139,6 +149,8 @@ struct cholespy_gpu_solver {
     bool has_matrix = false;
     float *r32 = nullptr;                 // FP32 right-hand side of correction solves
     size_t r32_elems = 0;
+    double *norms = nullptr;              // refinement stopping test: [0, 128) residual, [128, 256) b
+
     int grid_rows[2][2] = {{0, 0}, {0, 0}};   // [lower][k > 64]
     int grid_blocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // [forward][tile_slot]
     // Ticket queues per kernel: [0..1] analysis upper/lower, [2..3] row pass
@@ -327,6 +339,37 @@ int upload_plan_values(cholespy_gpu_solver *s, const BlockPlan &P) {
     return CHOLESPY_OK;
 }
 
+// Restores the all-sentinel state of every work buffer (the state a
+// completed solve leaves behind).
+int refill_work_buffers(cholespy_gpu_solver *s, hipStream_t st) {
+    if (s->work_elems) {
+        HIP_TRY(launch_fill_sentinel(s->y, s->work_elems, s->fbytes, st));
+        HIP_TRY(launch_fill_sentinel(s->z, s->work_elems, s->fbytes, st));
+        if (s->rbuf) HIP_TRY(launch_fill_sentinel(s->rbuf, s->work_elems, s->fbytes, st));
+    }
+    if (s->part && s->part_elems) HIP_TRY(launch_fill_sentinel(s->part, s->part_elems, s->fbytes, st));
+    if (s->spart && s->spart_elems) HIP_TRY(launch_fill_sentinel(s->spart, s->spart_elems, s->fbytes, st));
+    return CHOLESPY_OK;
+}
+
+// After a kernel of the handle gave up a wait (or the abort word was raised):
+// once the device is idle, clears the watchdog words and the ticket queues,
+// refills the work buffers and returns CHOLESPY_ERR_TIMEOUT, so the next call
+// starts from a clean state. CHOLESPY_OK when the words are clear.
+int report_give_up(cholespy_gpu_solver *s) {
+    if (s->hstatus[0] == 0 && s->hstatus[1] == 0) return CHOLESPY_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    const bool timed_out = s->hstatus[0] != 0;
+    s->hstatus[0] = s->hstatus[1] = 0;
+    HIP_TRY(hipMemset(s->qctr, 0, 6 * kQueueWords * sizeof(unsigned int)));
+    int rc = refill_work_buffers(s, nullptr);
+    if (rc != CHOLESPY_OK) return rc;
+    HIP_TRY(hipDeviceSynchronize());
+    s->fwd_pending = false;
+    return fail(CHOLESPY_ERR_TIMEOUT, timed_out ? "a solve kernel exceeded its bounded wait; its results were NaN"
+                                                : "the solver's abort word was raised; the last results may be NaN");
+}
+
 }  // namespace
 
 extern "C" {
@@ -378,8 +421,15 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
     s->n = n;
     s->fbytes = float_bytes;
     s->device = g_device;
-    hipError_t e = dev_alloc(&s->status, 2);
-    if (e == hipSuccess) e = hipMemset(s->status, 0, 2 * sizeof(int));
+    void *hs = nullptr;
+    hipError_t e = hipHostMalloc(&hs, 4 * sizeof(int), hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) {
+        s->hstatus = static_cast<volatile int *>(hs);
+        for (int i = 0; i < 4; ++i) s->hstatus[i] = 0;
+        void *dp = nullptr;
+        e = hipHostGetDevicePointer(&dp, hs, 0);
+        s->status = static_cast<int *>(dp);
+    }
     if (e == hipSuccess) e = dev_alloc(&s->perm, (size_t)n);
     if (e == hipSuccess) e = dev_alloc(&s->lvl1, (size_t)n);
     const size_t qwords = 6 * kQueueWords;
@@ -402,7 +452,16 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
 int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     if (!s) return CHOLESPY_OK;
     (void)hipDeviceSynchronize();
-    dev_free(s->status);
+    if (s->hstatus) {
+        // Nothing can raise from a destructor: report a give-up no solve call
+        // has reported yet the way the reference reports device errors
+        // (src/cuda_driver.cpp:48-56), on stderr.
+        if (s->hstatus[0] != 0)
+            std::fprintf(stderr, "cholespy_amd: a solve kernel exceeded its bounded wait (reported at destroy)\n");
+        (void)hipHostFree(const_cast<int *>(s->hstatus));
+    }
+    s->hstatus = nullptr;
+    s->status = nullptr;
     dev_free(s->perm);
     dev_free(s->lvl1);
     dev_free(s->qctr);
@@ -425,6 +484,7 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->acols);
     dev_free(s->avals);
     dev_free(s->r32);
+    dev_free(s->norms);
     dev_free(s->blk_sys);
     dev_free(s->sys_k);
     dev_free(s->sys_vbase);
@@ -501,7 +561,12 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     s->analysed[0] = s->analysed[1] = false;
 
     BlockPlanOptions opt = plan_options();
-    BlockPlan P = build_block_plan(n, colptr, rowidx, val, opt);
+    BlockPlan P;
+    try {
+        P = build_block_plan(n, colptr, rowidx, val, opt);
+    } catch (const std::exception &ex) {
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, ex.what());
+    }
     rc = upload_plan(s, P);
     if (rc != CHOLESPY_OK) return rc;
     // The structure stays on the host for refactorisations; the values are on the device.
@@ -574,18 +639,19 @@ int cholespy_solver_analyze(cholespy_gpu_solver *s, int lower, int *n_levels) {
     const int *rowptr = lower ? s->lrowptr : s->urowptr;
     const int *cols = lower ? s->lcols : s->ucols;
     HIP_TRY(hipMemset(s->lvl1, 0, (size_t)n * sizeof(int)));
-    HIP_TRY(hipMemset(s->status, 0, 2 * sizeof(int)));
+    // No kernel of this handle may be running while the watchdog words are
+    // cleared (one could be giving up right now).
+    HIP_TRY(hipDeviceSynchronize());
+    s->hstatus[0] = s->hstatus[1] = 0;
     const int cap = max_coresident_blocks(level_analysis_kernel(lower != 0), kRowBlock, 0);
     if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for level analysis");
     const int grid = std::max(1, std::min(cap, (n + 4 * kRowChunk - 1) / (4 * kRowChunk)));
     HIP_TRY(launch_level_analysis(lower != 0, n, rowptr, cols, s->lvl1, s->status, grid,
                                   s->queues[li], nullptr));
     std::vector<int> lvl1((size_t)n);
-    int status[2] = {0, 0};
     HIP_TRY(hipMemcpy(lvl1.data(), s->lvl1, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(status, s->status, sizeof(status), hipMemcpyDeviceToHost));
-    if (status[0] != 0) {
-        HIP_TRY(hipMemset(s->status, 0, 2 * sizeof(int)));
+    if (s->hstatus[0] != 0 || s->hstatus[1] != 0) {
+        s->hstatus[0] = s->hstatus[1] = 0;
         return fail(CHOLESPY_ERR_TIMEOUT, "level analysis exceeded its bounded wait");
     }
     // Stable counting sort of rows by level (src/cholesky_solver.cpp:348-367).
@@ -623,25 +689,20 @@ int cholespy_solver_get_levels(const cholespy_gpu_solver *s, int lower, int *lev
 
 int cholespy_solver_check(cholespy_gpu_solver *s) {
     if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
-    int status[2] = {0, 0};
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(status, s->status, sizeof(status), hipMemcpyDeviceToHost));
-    if (status[0] != 0 || status[1] != 0) {
-        HIP_TRY(hipMemset(s->status, 0, 2 * sizeof(int)));
-        HIP_TRY(hipMemset(s->qctr, 0, 6 * kQueueWords * sizeof(unsigned int)));
-        // Work buffers may hold partial results: restore the sentinel state.
-        if (s->work_elems) {
-            HIP_TRY(launch_fill_sentinel(s->y, s->work_elems, s->fbytes, nullptr));
-            HIP_TRY(launch_fill_sentinel(s->z, s->work_elems, s->fbytes, nullptr));
-            if (s->part && s->part_elems)
-                HIP_TRY(launch_fill_sentinel(s->part, s->part_elems, s->fbytes, nullptr));
-            if (s->spart && s->spart_elems)
-                HIP_TRY(launch_fill_sentinel(s->spart, s->spart_elems, s->fbytes, nullptr));
-            if (s->rbuf) HIP_TRY(launch_fill_sentinel(s->rbuf, s->work_elems, s->fbytes, nullptr));
-            HIP_TRY(hipDeviceSynchronize());
-        }
-        return fail(CHOLESPY_ERR_TIMEOUT, "a solve kernel exceeded its bounded wait");
-    }
+    return report_give_up(s);
+}
+
+int cholespy_solver_set_spin_limit(cholespy_gpu_solver *s, uint32_t limit) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    if (limit == 0) limit = kSpinLimit;
+    for (QueueState &q : s->queues) q.spin_limit = limit;
+    return CHOLESPY_OK;
+}
+
+int cholespy_solver_raise_abort(cholespy_gpu_solver *s) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    s->hstatus[1] = 1;
     return CHOLESPY_OK;
 }
 
@@ -795,11 +856,28 @@ int solve_impl(cholespy_gpu_solver *s, int fbytes, int k, const void *b, void *x
     if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "solve called before the factor was uploaded");
     if (k == 0 || s->n == 0) return CHOLESPY_OK;
     if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    int rc = prepare_buffers(s, k, st);
+    // A wait that gave up in an earlier call of this handle is reported here
+    // (RuntimeError in the binding); the handle is then clean again.
+    int rc = report_give_up(s);
     if (rc != CHOLESPY_OK) return rc;
-    return s->path == CHOLESPY_PATH_ROWS ? solve_rows(s, k, b, x, st, only_pass)
-                                         : solve_blocks(s, k, b, x, st, 0, only_pass);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = prepare_buffers(s, k, st);
+    if (rc != CHOLESPY_OK) return rc;
+    // Pass order. A forward pass publishes partial sums, gathered right-hand
+    // sides and y, and only the backward pass resets them: a forward pass
+    // (or a whole solve) that follows a forward pass with no backward pass
+    // in between starts from refilled buffers. A lone backward pass waits on
+    // z, which the forward pass resets: refill z before it.
+    if (only_pass != 1 && s->fwd_pending) {
+        rc = refill_work_buffers(s, st);
+        if (rc != CHOLESPY_OK) return rc;
+    } else if (only_pass == 1 && !s->fwd_pending) {
+        HIP_TRY(launch_fill_sentinel(s->z, s->work_elems, s->fbytes, st));
+    }
+    rc = s->path == CHOLESPY_PATH_ROWS ? solve_rows(s, k, b, x, st, only_pass)
+                                       : solve_blocks(s, k, b, x, st, 0, only_pass);
+    if (rc == CHOLESPY_OK) s->fwd_pending = only_pass == 0;
+    return rc;
 }
 
 }  // namespace
@@ -858,8 +936,19 @@ int cholespy_solver_upload_matrix(cholespy_gpu_solver *s, const int *colptr, con
     return CHOLESPY_OK;
 }
 
-int cholespy_solver_solve_refined(cholespy_gpu_solver *s, int k, const double *b, double *x,
-                                  int iterations, void *stream) {
+}  // extern "C"
+
+namespace {
+
+// Mixed-precision solve (both entry points): an FP32 block solve, then
+// corrections x += A^{-1} (b - A x). tol < 0: exactly `iterations`
+// corrections, all enqueued (no host synchronisation, graph-capturable).
+// tol >= 0: before each correction the residual's column norms come back to
+// the host, and refinement stops once max_j ||b_j - A x_j|| / ||b_j|| <= tol
+// or after `iterations` corrections; *steps and *residual report the
+// corrections applied and the last relative residual.
+int refined_impl(cholespy_gpu_solver *s, int k, const double *b, double *x, int iterations, double tol,
+                 void *stream, int *steps, double *residual) {
     if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
     if (s->fbytes != 4) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "refined solves need a float32 solver");
     if (!s->has_matrix) return fail(CHOLESPY_ERR_STATE, "refined solve called before upload_matrix");
@@ -868,25 +957,75 @@ int cholespy_solver_solve_refined(cholespy_gpu_solver *s, int k, const double *b
     if (k == 0 || s->n == 0) return CHOLESPY_OK;
     if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
     if (s->path != CHOLESPY_PATH_BLOCKS) return fail(CHOLESPY_ERR_STATE, "refined solves use the block path");
-    hipStream_t st = static_cast<hipStream_t>(stream);
-    int rc = prepare_buffers(s, k, st);
+    int rc = report_give_up(s);
     if (rc != CHOLESPY_OK) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rc = prepare_buffers(s, k, st);
+    if (rc != CHOLESPY_OK) return rc;
+    if (s->fwd_pending) {
+        rc = refill_work_buffers(s, st);
+        if (rc != CHOLESPY_OK) return rc;
+        s->fwd_pending = false;
+    }
     const size_t elems = (size_t)s->n * (size_t)k;
     if (elems > s->r32_elems) {
         dev_free(s->r32);
         HIP_TRY(dev_alloc(&s->r32, elems));
         s->r32_elems = elems;
     }
-    // x = A^{-1} b in FP32, then `iterations` corrections x += A^{-1}(b - A x).
+    // x = A^{-1} b in FP32, then the corrections x += A^{-1}(b - A x).
     HIP_TRY(launch_cast_to_f32(b, s->r32, elems, st));
     rc = solve_blocks(s, k, s->r32, x, st, 1);
     if (rc != CHOLESPY_OK) return rc;
-    for (int it = 0; it < iterations; ++it) {
-        HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, st));
+    if (tol < 0) {
+        for (int it = 0; it < iterations; ++it) {
+            HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, nullptr, st));
+            rc = solve_blocks(s, k, s->r32, x, st, 2);
+            if (rc != CHOLESPY_OK) return rc;
+        }
+        if (steps) *steps = iterations;
+        return CHOLESPY_OK;
+    }
+    if (!s->norms) HIP_TRY(dev_alloc(&s->norms, 2 * (size_t)kMaxRhs));
+    double *rn = s->norms, *bn = s->norms + kMaxRhs;
+    HIP_TRY(launch_column_norms2((size_t)s->n, k, b, bn, st));
+    std::vector<double> h(2 * (size_t)kMaxRhs, 0.0);
+    int it = 0;
+    double rel = 0.0;
+    for (;; ++it) {
+        HIP_TRY(launch_zero_f64(rn, k, st));
+        HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, rn, st));
+        HIP_TRY(hipMemcpyAsync(h.data(), s->norms, 2 * (size_t)kMaxRhs * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        rc = report_give_up(s);
+        if (rc != CHOLESPY_OK) return rc;
+        rel = 0.0;
+        for (int t = 0; t < k; ++t) {
+            const double r = std::sqrt(h[t]), bb = std::sqrt(h[kMaxRhs + t]);
+            rel = std::max(rel, bb > 0 ? r / bb : (r > 0 ? INFINITY : 0.0));
+        }
+        if (!(rel > tol) || it >= iterations) break;   // NaN residuals stop too
         rc = solve_blocks(s, k, s->r32, x, st, 2);
         if (rc != CHOLESPY_OK) return rc;
     }
+    if (steps) *steps = it;
+    if (residual) *residual = rel;
     return CHOLESPY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cholespy_solver_solve_refined(cholespy_gpu_solver *s, int k, const double *b, double *x,
+                                  int iterations, void *stream) {
+    return refined_impl(s, k, b, x, iterations, -1.0, stream, nullptr, nullptr);
+}
+
+int cholespy_solver_solve_refined_tol(cholespy_gpu_solver *s, int k, const double *b, double *x, double tol,
+                                      int max_steps, void *stream, int *steps, double *residual) {
+    if (!(tol >= 0)) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "tol must be non-negative");
+    return refined_impl(s, k, b, x, max_steps, tol, stream, steps, residual);
 }
 
 int cholespy_solver_solve_pass(cholespy_gpu_solver *s, int pass, int n_rhs, const void *b, void *x,
@@ -896,15 +1035,15 @@ int cholespy_solver_solve_pass(cholespy_gpu_solver *s, int pass, int n_rhs, cons
     return solve_impl(s, s->fbytes, n_rhs, b ? b : x, x ? x : const_cast<void *>(b), stream, pass);
 }
 
-int cholespy_entry_gradient(int64_t nnz, const int *rows, const int *cols, int n_rhs, const void *lam,
+int cholespy_entry_gradient(int n, int64_t nnz, const int *rows, const int *cols, int n_rhs, const void *lam,
                             const void *x, int vec_bytes, void *grad, int grad_bytes, void *stream) {
-    if (nnz < 0 || n_rhs < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative size");
+    if (n < 0 || nnz < 0 || n_rhs < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative size");
     if ((vec_bytes != 4 && vec_bytes != 8) || (grad_bytes != 4 && grad_bytes != 8))
         return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "element sizes must be 4 or 8");
     if (nnz == 0) return CHOLESPY_OK;
     if (!rows || !cols || !grad || (n_rhs > 0 && (!lam || !x)))
         return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null pointer");
-    HIP_TRY(launch_entry_gradient(nnz, rows, cols, n_rhs, lam, x, vec_bytes, grad, grad_bytes,
+    HIP_TRY(launch_entry_gradient(n, nnz, rows, cols, n_rhs, lam, x, vec_bytes, grad, grad_bytes,
                                   static_cast<hipStream_t>(stream)));
     return CHOLESPY_OK;
 }
@@ -1004,14 +1143,21 @@ int cholespy_batch_finalize(cholespy_gpu_solver *s) {
     {
         const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
         std::vector<std::thread> pool;
+        std::vector<std::string> errors((size_t)ns);
         for (unsigned w = 0; w < hw; ++w)
             pool.emplace_back([&, w]() {
                 for (int i = (int)w; i < ns; i += (int)hw) {
                     const auto &h = s->pending[i];
-                    plans[i] = build_block_plan(s->sys_n[i], h.colptr.data(), h.rowidx.data(), h.val.data(), opt);
+                    try {
+                        plans[i] = build_block_plan(s->sys_n[i], h.colptr.data(), h.rowidx.data(), h.val.data(), opt);
+                    } catch (const std::exception &ex) {
+                        errors[i] = ex.what();
+                    }
                 }
             });
         for (auto &t : pool) t.join();
+        for (const std::string &e : errors)
+            if (!e.empty()) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, e);
     }
     // Merge: blocks, slots, contributions and dense values concatenated with
     // offsets; tasks of all systems interleaved by block level.
@@ -1149,8 +1295,13 @@ int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void
     }
     if (elems == 0) return CHOLESPY_OK;
     if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
+    int rc = report_give_up(s);
+    if (rc != CHOLESPY_OK) return rc;
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (ks != s->last_k) {
+        // Solves already queued on the caller's stream read the per-system
+        // arrays and buffers about to be rewritten: let them finish first.
+        HIP_TRY(hipStreamSynchronize(st));
         std::vector<long long> vb((size_t)ns), pb((size_t)ns), sb((size_t)ns);
         long long voff = 0, poff = 0, soff = 0;
         for (int i = 0; i < ns; ++i) {

@@ -15,7 +15,12 @@
 // Bounded waits. Every spin loop counts its polls; past a limit it records
 // CHOLESPY_ERR_TIMEOUT in the status word, raises the abort word and stops
 // waiting, and every other spinning lane sees the abort word and stops too,
-// so a broken schedule ends the kernel instead of hanging the device.
+// so a broken schedule ends the kernel instead of hanging the device. A wait
+// that gives up returns a quiet NaN, so the results it feeds are NaN, never
+// finite and wrong. The two status words live in host memory mapped into
+// the device (fine-grained, coherent): kernels only store to them, with
+// system-scope stores on the give-up path, and read the abort word every
+// 256 polls of a long wait; the host reads them with no device call.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -81,24 +86,34 @@ __device__ __forceinline__ void store_int(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// status[0] = error code (first writer wins), status[1] = abort flag.
+// status[0] = error code, status[1] = abort flag, in mapped host memory.
+// Plain system-scope stores and loads only (no read-modify-write on host
+// memory): every writer stores the same code.
+__device__ __forceinline__ int load_status(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 __device__ __forceinline__ void raise_timeout(int *status) {
-    int expected = 0;
-    __hip_atomic_compare_exchange_strong(status, &expected, kStatusTimeout, __ATOMIC_RELAXED,
-                                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    store_int(status + 1, 1);
+    __hip_atomic_store(status, kStatusTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Default poll budget of one wait (about 10 s of polling); a handle may set a
+// smaller one (cholespy_solver_set_spin_limit) to force the give-up path.
 constexpr uint32_t kSpinLimit = 1u << 23;
 #ifndef CHOLESPY_SPIN_SLEEP
 #define CHOLESPY_SPIN_SLEEP 1
 #endif
 
-// Waits until *p is no longer the sentinel and returns the value. On
-// timeout or abort returns 0 (the kernel's result is then invalid and the
-// status word says so).
 template <typename F>
-__device__ __forceinline__ F wait_value(const F *p, int *status) {
+__device__ __forceinline__ F give_up_value() {
+    return __builtin_nan("");
+}
+
+// Waits until *p is no longer the sentinel and returns the value. After
+// `limit` polls it records the timeout and returns a quiet NaN; it also
+// returns a quiet NaN when it sees the abort word (checked every 256 polls).
+template <typename F>
+__device__ __forceinline__ F wait_value(const F *p, int *status, uint32_t limit) {
     using U = typename Bits<F>::U;
     U u = load_bits(p);
     if (u != Bits<F>::sentinel) return from_bits<F>(u);
@@ -107,18 +122,17 @@ __device__ __forceinline__ F wait_value(const F *p, int *status) {
         __builtin_amdgcn_s_sleep(CHOLESPY_SPIN_SLEEP);
         u = load_bits(p);
         if (u != Bits<F>::sentinel) return from_bits<F>(u);
-        if ((++spins & 255u) == 0) {
-            if (load_int(status + 1)) return F(0);
-            if (spins >= kSpinLimit) {
-                raise_timeout(status);
-                return F(0);
-            }
+        if (++spins >= limit) {
+            raise_timeout(status);
+            return give_up_value<F>();
         }
+        if ((spins & 255u) == 0 && load_status(status + 1)) return give_up_value<F>();
     }
 }
 
-// Waits until *p is non-zero and returns it (same watchdog as wait_value).
-__device__ __forceinline__ int wait_nonzero(const int *p, int *status) {
+// Waits until *p is non-zero and returns it (same watchdog as wait_value;
+// returns 1 when it gives up, and the status word says so).
+__device__ __forceinline__ int wait_nonzero(const int *p, int *status, uint32_t limit) {
     int v = load_int(p);
     if (v != 0) return v;
     uint32_t spins = 0;
@@ -126,13 +140,11 @@ __device__ __forceinline__ int wait_nonzero(const int *p, int *status) {
         __builtin_amdgcn_s_sleep(1);
         v = load_int(p);
         if (v != 0) return v;
-        if ((++spins & 255u) == 0) {
-            if (load_int(status + 1)) return 1;
-            if (spins >= kSpinLimit) {
-                raise_timeout(status);
-                return 1;
-            }
+        if (++spins >= limit) {
+            raise_timeout(status);
+            return 1;
         }
+        if ((spins & 255u) == 0 && load_status(status + 1)) return 1;
     }
 }
 
@@ -153,6 +165,7 @@ constexpr int kQueueStride = 16;   // 128 bytes between counters
 struct QueueArgs {
     unsigned int *ctr;    // nq counters kQueueStride apart, then the done counter
     int nq;
+    uint32_t spin_limit;  // poll budget of every bounded wait of the launch
 };
 
 // Next ticket of queue q, or -1 when the queue is exhausted (n tickets in all).

@@ -16,6 +16,7 @@ struct QueueArgs;   // common.hpp
 struct QueueState {
     unsigned int *ctr = nullptr;
     int nq = 16;
+    uint32_t spin_limit = 1u << 23;   // poll budget of each bounded wait (common.hpp kSpinLimit)
     void fill(QueueArgs &qa) const;
 };
 constexpr size_t kQueueWords = 16 * 16 + 16;   // counters + done word (padded)
@@ -133,11 +134,17 @@ hipError_t launch_block_pass(int float_bytes, bool forward, const BlockPassBuffe
 
 // Mixed-precision refinement helpers (refine.hip).
 hipError_t launch_cast_to_f32(const double *in, float *out, size_t count, hipStream_t stream);
+// With norm2 (k doubles, zeroed by the caller) not null, also adds the
+// residual's squared column norms into it.
 hipError_t launch_residual_f64(int n, int k, const int *rowptr, const int *cols, const double *vals,
-                               const double *b, const double *x, float *r32, hipStream_t stream);
+                               const double *b, const double *x, float *r32, double *norm2, hipStream_t stream);
+// norm2[t] = sum_i v[i,t]^2 for a row-major (rows x k) FP64 array.
+hipError_t launch_column_norms2(size_t rows, int k, const double *v, double *norm2, hipStream_t stream);
+hipError_t launch_zero_f64(double *p, int count, hipStream_t stream);
 
 // dL/d(stored entries of A) from the adjoint lam and the solution x (grad.hip).
-hipError_t launch_entry_gradient(int64_t nnz, const int *rows, const int *cols, int k, const void *lam,
+// Entries with an index outside [0, n) get NaN and are not read through.
+hipError_t launch_entry_gradient(int n, int64_t nnz, const int *rows, const int *cols, int k, const void *lam,
                                  const void *x, int vec_bytes, void *grad, int grad_bytes,
                                  hipStream_t stream);
 

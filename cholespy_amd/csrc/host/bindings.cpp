@@ -361,6 +361,10 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_n_levels", [](const Class &self, bool lower) { return self.n_levels(lower); },
              py::arg("lower"))
         .def("_check", [](const Class &self) { self.check(); })
+        // Watchdog test hooks (cholespy_solver_set_spin_limit / _raise_abort).
+        .def("_set_spin_limit", [](Class &self, uint32_t limit) { throw_on_error(cholespy_solver_set_spin_limit(self.gpu(), limit)); },
+             py::arg("limit"))
+        .def("_raise_abort", [](Class &self) { throw_on_error(cholespy_solver_raise_abort(self.gpu())); })
         .def("_set_trace", [](Class &self, bool on) { throw_on_error(cholespy_solver_set_trace(self.gpu(), on ? 1 : 0)); },
              py::arg("on"))
         .def("_plan_arrays", [](const Class &self) { return plan_arrays(self.gpu()); })
@@ -408,6 +412,33 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
                                    static_cast<double *>(x.data()), iterations);
             },
             py::arg("b"), py::arg("x"), py::arg("iterations") = 2)
+        .def(
+            "_solve_refined_tol",
+            [](Class &self, py::object b_o, py::object x_o, double tol, int max_steps) {
+                DLView b(b_o), x(x_o);
+                const auto good = [](const DLView &v) {
+                    return v.dtype().code == dl::kFloat && v.dtype().bits == 64 && v.dtype().lanes == 1 &&
+                           v.c_contig() && v.device() == Dev::GPU;
+                };
+                if (!good(b) || !good(x) || x.readonly())
+                    throw py::type_error("_solve_refined_tol(): b and x must be C-contiguous float64 GPU arrays.");
+                if (b.ndim() != 1 && b.ndim() != 2)
+                    throw std::invalid_argument("Expected 1D or 2D tensors as input.");
+                if (b.ndim() != x.ndim() || b.shape(0) != x.shape(0) ||
+                    (b.ndim() == 2 && b.shape(1) != x.shape(1)))
+                    throw std::invalid_argument("x and b should have the same dimensions.");
+                if (b.shape(0) != self.n()) throw std::invalid_argument("b and x should have n_rows rows.");
+                const int k = b.ndim() == 2 ? (int)b.shape(1) : 1;
+                if (k > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
+                if (!self.gpu()) throw std::invalid_argument("Refined solves only exist for GPU solvers.");
+                int steps = 0;
+                double residual = 0.0;
+                throw_on_error(cholespy_solver_solve_refined_tol(self.gpu(), k, static_cast<const double *>(b.data()),
+                                                                 static_cast<double *>(x.data()), tol, max_steps,
+                                                                 nullptr, &steps, &residual));
+                return py::make_tuple(steps, residual);
+            },
+            py::arg("b"), py::arg("x"), py::arg("tol"), py::arg("max_steps") = 10)
         .def("_plan_info",
              [](const Class &self) {
                  const std::vector<int64_t> v = self.plan_info();
@@ -522,7 +553,7 @@ void entry_gradient(py::object rows_o, py::object cols_o, py::object lam_o, py::
         lam.size() != x.size())
         throw std::invalid_argument("_entry_gradient(): inconsistent sizes.");
     const int k = x.ndim() == 2 ? (int)x.shape(1) : 1;
-    throw_on_error(cholespy_entry_gradient(rows.size(), static_cast<const int *>(rows.data()),
+    throw_on_error(cholespy_entry_gradient((int)x.shape(0), rows.size(), static_cast<const int *>(rows.data()),
                                            static_cast<const int *>(cols.data()), k, lam.data(), x.data(),
                                            lam.dtype().bits / 8, grad.data(), grad.dtype().bits / 8,
                                            nullptr));

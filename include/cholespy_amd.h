@@ -158,7 +158,12 @@ int cholespy_solver_solve_f64(cholespy_gpu_solver *solver, int n_rhs, const doub
  * pass 0 then pass 1 with the same n_rhs equals one solve. Replaces
  * launch_kernel(lower, x) (src/cholesky_solver.cpp:378-413); used to time
  * the two kernels separately. b is ignored by pass 1 and x by pass 0 (either
- * may then be NULL). */
+ * may then be NULL). Any order of calls is valid, as with the reference,
+ * which clears its flags at every launch (src/cholesky_solver.cpp:380-382):
+ * a forward pass (or a whole solve) that follows a forward pass with no
+ * backward pass in between, and a backward pass with no forward pass before
+ * it, first restore the handle's work buffers (extra fill kernels on
+ * `stream`). A backward pass reads the y of the last forward pass. */
 int cholespy_solver_solve_pass(cholespy_gpu_solver *solver, int pass, int n_rhs, const void *b,
                                void *x, void *stream);
 
@@ -175,6 +180,17 @@ int cholespy_solver_upload_matrix(cholespy_gpu_solver *solver, const int *colptr
  * reference (its CholeskySolverF solves in FP32 only). */
 int cholespy_solver_solve_refined(cholespy_gpu_solver *solver, int n_rhs, const double *b,
                                   double *x, int iterations, void *stream);
+
+/* The same, refined until the relative residual
+ * max_j ||b_j - A x_j||_2 / ||b_j||_2 (FP64, on the device) is <= tol, or
+ * after max_steps corrections. The residual norms come back to the host
+ * before each correction (one synchronisation of `stream` per step, so this
+ * variant cannot be captured in a graph). *steps (corrections applied) and
+ * *residual (the last relative residual) are written when not NULL. No
+ * counterpart in the reference. */
+int cholespy_solver_solve_refined_tol(cholespy_gpu_solver *solver, int n_rhs, const double *b,
+                                      double *x, double tol, int max_steps, void *stream, int *steps,
+                                      double *residual);
 
 /* Batches of independent systems solved by one launch per pass (no
  * counterpart in the reference, which solves one system per object).
@@ -216,14 +232,17 @@ int cholespy_solver_plan_arrays(const cholespy_gpu_solver *solver, int which, in
                                 int64_t *size);
 
 /* Gradient of a loss with respect to the caller's stored matrix entries
- * (rows[e], cols[e], e < nnz, device int32 arrays) for x = A^{-1} b, given
+ * (rows[e], cols[e], e < nnz, device int32 arrays, indices in [0, n)) of an
+ * n x n matrix A, for x = A^{-1} b, given
  * x and the adjoint lam = A^{-1} dL/dx (device, row-major n x n_rhs,
  * vec_bytes = 4 or 8): grad[e] = -sum_t (lam[i,t] x[j,t] + lam[j,t] x[i,t])
  * for i > j, -sum_t lam[i,t] x[i,t] for i == j, 0 for i < j (the strict
  * upper triangle is never read: CHOLMOD stype = -1,
  * src/cholesky_solver.cpp:211-220). grad has grad_bytes = 4 or 8 per
- * element. No counterpart in the reference (it has no backward). */
-int cholespy_entry_gradient(int64_t nnz, const int *rows, const int *cols, int n_rhs,
+ * element. An entry with an index outside [0, n) is never read through; its
+ * gradient is NaN (check the range first; the Python layer raises). No
+ * counterpart in the reference (it has no backward). */
+int cholespy_entry_gradient(int n, int64_t nnz, const int *rows, const int *cols, int n_rhs,
                             const void *lam, const void *x, int vec_bytes, void *grad,
                             int grad_bytes, void *stream);
 
@@ -236,11 +255,28 @@ int cholespy_solver_set_trace(cholespy_gpu_solver *solver, int enable);
 int cholespy_solver_get_trace(const cholespy_gpu_solver *solver, int forward, uint64_t *out,
                               int64_t words);
 
-/* Reads and clears the handle's device status word (the bounded-wait
- * watchdog of the solve and analysis kernels). Synchronises the device.
- * Returns CHOLESPY_ERR_TIMEOUT if a kernel gave up waiting since the last
- * call, CHOLESPY_OK otherwise. */
+/* Watchdog. Every wait of the solve and analysis kernels is bounded: a wait
+ * that runs out of polls records CHOLESPY_ERR_TIMEOUT in the handle's status
+ * words (host memory mapped into the device) and raises the handle's abort
+ * word, and every wait that gives up returns NaN, so the results of that
+ * call are NaN rather than finite and wrong. The next solve, refined solve,
+ * pass or batch solve on the handle (or cholespy_solver_check) reports it:
+ * it waits for the device, clears the words, restores the work buffers and
+ * returns CHOLESPY_ERR_TIMEOUT without solving, and the call after it runs
+ * normally. The reference prints driver errors and continues
+ * (src/cuda_driver.cpp:48-56); its spin-waits have no bound.
+ *
+ * cholespy_solver_check synchronises the device and reports as above
+ * (CHOLESPY_OK when nothing gave up). */
 int cholespy_solver_check(cholespy_gpu_solver *solver);
+
+/* Test hooks of the watchdog (no counterpart in the reference): set the
+ * poll budget of every wait of the handle's later launches (0 = the default,
+ * about 10 s), and raise the handle's abort word from the host as a give-up
+ * would. Tests force the give-up path with these, so no kernel has to hang
+ * for real. */
+int cholespy_solver_set_spin_limit(cholespy_gpu_solver *solver, uint32_t limit);
+int cholespy_solver_raise_abort(cholespy_gpu_solver *solver);
 
 #ifdef __cplusplus
 }

@@ -166,6 +166,32 @@ def host_solve(factor: dict, b):
     return np.ascontiguousarray(bx).reshape(b.shape)
 
 
+def host_solve_parallel(factor: dict, b, threads: int):
+    """host_solve with the right-hand-side columns shared out over `threads`
+    host threads (each column is an independent solve; ctypes releases the
+    GIL for the C call). Same result as host_solve, bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+    b = np.asarray(b, dtype=np.float64)
+    n = len(factor["perm"])
+    b2 = b.reshape(n, -1)
+    k = b2.shape[1]
+    bx = np.asfortranarray(b2).copy(order="F")
+    colptr, rowidx, perm = _i32(factor["colptr"]), _i32(factor["rowidx"]), _i32(factor["perm"])
+    val = np.ascontiguousarray(factor["val"], dtype=np.float64)
+    L = lib()
+    base = bx.ctypes.data
+    cols = [list(range(t, k, threads)) for t in range(min(threads, k))]
+
+    def run(ts):
+        for t in ts:
+            L.oracle_host_solve(n, 1, _p(colptr), _p(rowidx), _p(val), _p(perm),
+                                ctypes.c_void_p(base + 8 * n * t))
+
+    with ThreadPoolExecutor(max_workers=len(cols)) as ex:
+        list(ex.map(run, cols))
+    return np.ascontiguousarray(bx).reshape(b.shape)
+
+
 # ---------------------------------------------------------------- input semantics
 def normalized_lower_csc(n, ii, jj, x, kind: str):
     """(colptr, rowidx, val) of the lower triangle of A as CHOLMOD reads it

@@ -201,3 +201,34 @@ def test_split_reductions_match_oracle(case, cls, dtype, split_entries, monkeypa
         ref = oracle.solve(fac, b, dtype)
         err = np.abs(xt.cpu().numpy() - ref).max() / np.abs(ref).max()
         assert err < TOL[dtype], (case, k, err)
+
+
+@pytest.mark.parametrize("case", ["ico5", "grid64"])
+def test_refined_to_tolerance(case):
+    """Refinement until the device-computed relative residual is <= tol: the
+    reported residual matches SciPy's on the host, the error against the
+    FP64 host solve is <= 1e-10, and no more steps are used than the fixed
+    schedule needs for the same residual."""
+    import scipy.sparse as sp
+    n, values, idx = CASES[case]()
+    solver = _gpu_solver(CholeskySolverF, n, values, idx)
+    A = sp.coo_matrix((values, (idx[0], idx[1])), shape=(n, n)).tocsr()
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal((n, 3))
+    host = oracle.host_solve(solver._factor(), b)
+    bt = torch.tensor(b, device=DEV)
+    xt = torch.zeros_like(bt)
+    steps, res = solver._solve_refined_tol(bt, xt, 1e-12, 8)
+    torch.cuda.synchronize()
+    solver._check()
+    x = xt.cpu().numpy()
+    res_host = float((np.linalg.norm(b - A @ x, axis=0) / np.linalg.norm(b, axis=0)).max())
+    assert res <= 1e-12 and abs(res - res_host) <= 0.05 * res_host + 1e-17, (res, res_host)
+    assert np.linalg.norm(x - host) / np.linalg.norm(host) <= 1e-10
+    assert 1 <= steps <= 3
+    # one step fewer does not reach the bar
+    xt2 = torch.zeros_like(bt)
+    solver._solve_refined(bt, xt2, steps - 1)
+    torch.cuda.synchronize()
+    x2 = xt2.cpu().numpy()
+    assert float((np.linalg.norm(b - A @ x2, axis=0) / np.linalg.norm(b, axis=0)).max()) > 1e-12

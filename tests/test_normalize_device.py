@@ -1,11 +1,13 @@
 """Device-side input normalisation (cholespy_normalize_input): GPU inputs to
-the constructor are reduced to the lower CSC on the GPU. The result must be
-bit-identical to the host normalisation of the same arrays (CPU-mode solver
-built from them), which follows the reference's host path
-(src/cholesky_solver.cpp:8-149: sort, sum duplicates in input order; stype =
--1 keeps row >= col), with the same errors for invalid indices and pointers
-(src/main.cpp:28-45 checks sizes before; the index checks are the host
-normalisation's)."""
+the constructor are reduced to the lower CSC on the GPU. The result must
+equal the oracle's restatement of the reference's host path
+(oracle.normalized_lower_csc: src/cholesky_solver.cpp:8-149, sort and sum
+duplicates; CHOLMOD stype = -1 keeps row >= col, 211-220): pattern bit-exact,
+values to the rounding of a duplicate sum in another order. It must also be
+bit-identical to this project's host normalisation of the same arrays
+(CPU-mode solver built from them), with the same errors for invalid indices
+and pointers (src/main.cpp:28-45 checks sizes before; the index checks are
+the host normalisation's)."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -13,6 +15,7 @@ import scipy.sparse as sp
 torch = pytest.importorskip("torch")
 
 from cholespy_amd import CholeskySolverD, CholeskySolverF, MatrixType, workloads
+from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -52,6 +55,15 @@ def assert_same(got, want):
         assert np.array_equal(got[key], want[key]), key
 
 
+def assert_matches_oracle(got, n, ii, jj, vv, kind):
+    colptr, rowidx, val = oracle.normalized_lower_csc(n, ii, jj, vv, kind)
+    assert np.array_equal(got["colptr"], colptr)
+    assert np.array_equal(got["rowidx"], rowidx)
+    # duplicates are summed in input order here; SciPy may add them in
+    # another order: agreement to a few units in the last place of the sum
+    np.testing.assert_allclose(got["val"], val, rtol=1e-14, atol=1e-15 * np.abs(val).max())
+
+
 @pytest.mark.parametrize("idx_dtype", [torch.int32, torch.int64])
 @pytest.mark.parametrize("val_dtype", [torch.float32, torch.float64])
 def test_coo_matches_host(idx_dtype, val_dtype):
@@ -60,6 +72,7 @@ def test_coo_matches_host(idx_dtype, val_dtype):
     want = host_matrix(CholeskySolverD, n, ii, jj, vcast, MatrixType.COO)
     s = CholeskySolverD(n, gpu(ii, idx_dtype), gpu(jj, idx_dtype), gpu(vv, val_dtype), MatrixType.COO)
     assert_same(s._matrix(), want)
+    assert_matches_oracle(s._matrix(), n, ii, jj, vcast, "COO")
 
 
 @pytest.mark.parametrize("kind", ["CSR", "CSC"])
@@ -79,6 +92,32 @@ def test_compressed_matches_host(kind, idx_dtype):
     want = host_matrix(CholeskySolverD, n, ptr, ind, val, mt)
     s = CholeskySolverF(n, gpu(ptr, idx_dtype), gpu(ind, idx_dtype), gpu(val), mt)
     assert_same(s._matrix(), want)
+    assert_matches_oracle(s._matrix(), n, ptr, ind, val, kind)
+
+
+@pytest.mark.parametrize("kind", ["COO", "CSR", "CSC"])
+@pytest.mark.parametrize("val_dtype", [torch.float32, torch.float64])
+def test_device_normalisation_matches_oracle(kind, val_dtype):
+    """Straight against the oracle (not the host path): duplicates spread
+    over the input, entries above the diagonal, int64 indices, both value
+    widths, every matrix type."""
+    n, ii, jj, vv = noisy_coo(level=2, seed=4)
+    vv = torch.as_tensor(vv).to(val_dtype).double().numpy()
+    if kind == "COO":
+        a, b_, c = ii, jj, vv
+    else:
+        A = sp.coo_matrix((vv, (ii, jj)), shape=(n, n))
+        # compressed form with duplicates kept (no sum_duplicates): rows /
+        # columns sorted by their outer index, inner order as given
+        outer, inner = (ii, jj) if kind == "CSR" else (jj, ii)
+        order = np.argsort(outer, kind="stable")
+        ptr = np.zeros(n + 1, dtype=np.int64)
+        np.add.at(ptr, outer + 1, 1)
+        a, b_, c = np.cumsum(ptr), inner[order], vv[order]
+        del A
+    mt = getattr(MatrixType, kind)
+    s = CholeskySolverD(n, gpu(a, torch.int64), gpu(b_, torch.int64), gpu(c, val_dtype), mt)
+    assert_matches_oracle(s._matrix(), n, a, b_, c, kind)
 
 
 def test_solution_matches_host_path():

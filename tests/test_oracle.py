@@ -181,3 +181,14 @@ def test_spsolve_end_to_end_small():
     A = sp.coo_matrix((g["coo_values"], (g["coo_idx"][0], g["coo_idx"][1])), shape=(n, n)).toarray()
     x = oracle.host_solve(factor_of(g), g["b"])
     assert np.allclose(x, np.linalg.solve(A, g["b"]), rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["ico3", "grid64"])
+def test_parallel_host_solve_is_the_same_solve(name):
+    """The all-cores CPU baseline shares the columns out over threads; every
+    column is the same sequential solve, so the result is bit-identical."""
+    g = load(name)
+    rng = np.random.default_rng(9)
+    b = rng.standard_normal((int(g["n"]), 5))
+    f = factor_of(g)
+    assert np.array_equal(oracle.host_solve_parallel(f, b, 3), oracle.host_solve(f, b))

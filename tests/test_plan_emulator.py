@@ -76,6 +76,32 @@ OPTIONS = [
 ]
 
 
+@pytest.fixture(scope="module")
+def grid3d_factor(tmp_path_factory):
+    """32^3 7-point grid: its elimination tree has a 1,411-row chain, so the
+    forced caps below give one block of more than 1024 rows."""
+    m = 32
+    values, idx = workloads.grid3d_laplacian(m, 1.0)
+    fac = CholeskySolverD(m ** 3, idx[0], idx[1], values, MatrixType.COO)._factor()
+    path = tmp_path_factory.mktemp("factors3d") / "grid32.bin"
+    with open(path, "wb") as fh:
+        fh.write(np.array([m ** 3, len(fac["rowidx"])], dtype=np.int64).tobytes())
+        fh.write(np.ascontiguousarray(fac["colptr"], dtype=np.int32).tobytes())
+        fh.write(np.ascontiguousarray(fac["rowidx"], dtype=np.int32).tobytes())
+        fh.write(np.ascontiguousarray(fac["val"], dtype=np.float64).tobytes())
+    return path
+
+
+@pytest.mark.parametrize("opts", [["max_block=2048"], ["max_block=4096"],
+                                  ["max_block=4096", "split_entries=512"]],
+                         ids=lambda o: ",".join(o))
+@pytest.mark.parametrize("task_list", ["narrow", "wide"])
+def test_long_chain_blocks_run_in_ticket_order(emulator, grid3d_factor, task_list, opts):
+    res = subprocess.run([str(emulator), str(grid3d_factor), task_list, *opts],
+                         capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0 and res.stdout.startswith("ok"), res.stdout + res.stderr
+
+
 @pytest.mark.parametrize("opts", OPTIONS, ids=lambda o: ",".join(o) or "default")
 @pytest.mark.parametrize("task_list", ["narrow", "wide"])
 @pytest.mark.parametrize("case", list(CASES))
