@@ -14,6 +14,8 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <atomic>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -24,11 +26,27 @@
 
 using namespace cholespy_amd;
 
+// Work memory of cholespy_solver_solve_noalloc: one array per GPU, in the
+// library's code object, so that entry point needs no allocation. Its
+// device address per GPU comes from hipGetSymbolAddress (the name itself
+// is not a device address on the host).
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 namespace {
 
 thread_local std::string g_error;
 bool g_initialised = false;
 int g_device = -1;
+
+// Per device: the scratch array's device address (null until the first
+// no-allocation solve on that device) and the layout its work buffers hold
+// in the all-sentinel state (0 = unknown: refill before use).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+uint64_t g_scratch_layout[kMaxDevices] = {};
+std::atomic<uint64_t> g_next_uid{1};
 
 int fail(int code, const std::string &msg) {
     g_error = msg;
@@ -106,6 +124,7 @@ int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<in
 }  // namespace
 
 struct cholespy_gpu_solver {
+    uint64_t uid = 0;   // unique per handle (keys the scratch layout)
     int n = 0;
     int fbytes = 4;
     int64_t nnz = 0;
@@ -361,6 +380,12 @@ int report_give_up(cholespy_gpu_solver *s) {
     HIP_TRY(hipDeviceSynchronize());
     const bool timed_out = s->hstatus[0] != 0;
     s->hstatus[0] = s->hstatus[1] = 0;
+    {
+        // A no-allocation solve may have given up with the scratch array's
+        // buffers half consumed: refill them before their next use.
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        for (uint64_t &l : g_scratch_layout) l = 0;
+    }
     HIP_TRY(hipMemset(s->qctr, 0, 6 * kQueueWords * sizeof(unsigned int)));
     int rc = refill_work_buffers(s, nullptr);
     if (rc != CHOLESPY_OK) return rc;
@@ -418,6 +443,7 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
     int rc = cholespy_init();
     if (rc != CHOLESPY_OK) return rc;
     auto *s = new cholespy_gpu_solver();
+    s->uid = g_next_uid.fetch_add(1);
     s->n = n;
     s->fbytes = float_bytes;
     s->device = g_device;
@@ -781,8 +807,14 @@ int solve_rows(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStream_
     return CHOLESPY_OK;
 }
 
+// Work buffers of one block solve (the handle's own, or the library's
+// scratch array for cholespy_solver_solve_noalloc).
+struct WorkSet {
+    void *y, *z, *part, *spart, *rbuf;
+};
+
 int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStream_t st, int xmode = 0,
-                 int only_pass = -1) {
+                 int only_pass = -1, const WorkSet *work = nullptr) {
     const int fbytes = s->fbytes;
     const DeviceBlockPlan &d = s->plan;
     BlockPassBuffers bf;
@@ -802,11 +834,13 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     bf.perm = s->perm;
     bf.b = b;
     bf.x = x;
-    bf.y = s->y;
-    bf.z = s->z;
-    bf.part = s->part;
-    bf.spart = s->spart;
-    bf.rbuf = s->rbuf;
+    const WorkSet own{s->y, s->z, s->part, s->spart, s->rbuf};
+    const WorkSet &w = work ? *work : own;
+    bf.y = w.y;
+    bf.z = w.z;
+    bf.part = w.part;
+    bf.spart = w.spart;
+    bf.rbuf = w.rbuf;
     bf.status = s->status;
     bf.xmode = xmode;
     if (s->is_batch) {
@@ -1391,6 +1425,64 @@ int cholespy_solver_solve_f32(cholespy_gpu_solver *s, int n_rhs, const float *b,
 int cholespy_solver_solve_f64(cholespy_gpu_solver *s, int n_rhs, const double *b, double *x,
                               void *stream) {
     return solve_impl(s, 8, n_rhs, b, x, stream);
+}
+
+int cholespy_solver_solve_noalloc(cholespy_gpu_solver *s, int n_rhs, const void *b, void *x, void *stream) {
+    if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
+    if (s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "batch handles solve with cholespy_batch_solve_*");
+    if (n_rhs > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "The number of RHS should be less than 128.");
+    if (n_rhs < 0) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "negative number of RHS");
+    if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "solve called before the factor was uploaded");
+    if (s->path != CHOLESPY_PATH_BLOCKS)
+        return fail(CHOLESPY_ERR_STATE, "the no-allocation solve runs the block schedule only");
+    if (n_rhs == 0 || s->n == 0) return CHOLESPY_OK;
+    if (!b || !x) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null b or x");
+    int rc = report_give_up(s);
+    if (rc != CHOLESPY_OK) return rc;
+    // Layout in the scratch array: y, z, rbuf (n x k each), partial slots,
+    // split rows, each region 256-byte aligned.
+    const size_t fb = (size_t)s->fbytes, k = (size_t)n_rhs;
+    const auto region = [](size_t bytes) { return (bytes + 255) & ~size_t(255); };
+    const size_t vec = region((size_t)s->n * k * fb), part = region((size_t)s->plan.nslots * k * fb),
+                 spart = region((size_t)s->plan.nsplit_rows * k * fb);
+    const size_t need = 3 * vec + part + spart;
+    if (need > kScratchBytes)
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT,
+                    "the no-allocation solve needs " + std::to_string(need) + " bytes of work memory for this system "
+                    "and n_rhs; its fixed array holds " + std::to_string(kScratchBytes) +
+                    " (use cholespy_solver_solve_f32/f64)");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "device index out of range");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        // Once per device, with that device current: the array's address there.
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CHOLESPY_ERR_HIP, "hipGetSymbolAddress returned a null address for the scratch array");
+        g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+    }
+    unsigned char *base = g_scratch_ptr[dev];
+    const WorkSet w{base, base + vec, base + 3 * vec, base + 3 * vec + part, base + 2 * vec};
+    // The array keeps the last call's contents. The kernels leave every
+    // buffer they wait on in the sentinel state, so a call with the same
+    // layout (handle, n_rhs) as the last one on this device starts as is;
+    // any other refills its regions with fill kernels first.
+    const uint64_t layout = (s->uid << 16) | (k << 4) | fb;
+    if (g_scratch_layout[dev] != layout) {
+        const size_t ne = (size_t)s->n * k;
+        HIP_TRY(launch_fill_sentinel(w.y, ne, s->fbytes, st));
+        HIP_TRY(launch_fill_sentinel(w.z, ne, s->fbytes, st));
+        HIP_TRY(launch_fill_sentinel(w.rbuf, ne, s->fbytes, st));
+        if (s->plan.nslots) HIP_TRY(launch_fill_sentinel(w.part, (size_t)s->plan.nslots * k, s->fbytes, st));
+        if (s->plan.nsplit_rows)
+            HIP_TRY(launch_fill_sentinel(w.spart, (size_t)s->plan.nsplit_rows * k, s->fbytes, st));
+        g_scratch_layout[dev] = layout;
+    }
+    rc = solve_blocks(s, n_rhs, b, x, st, 0, -1, &w);
+    if (rc != CHOLESPY_OK) g_scratch_layout[dev] = 0;
+    return rc;
 }
 
 }  // extern "C"

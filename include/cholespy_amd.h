@@ -153,6 +153,27 @@ int cholespy_solver_solve_f32(cholespy_gpu_solver *solver, int n_rhs, const floa
 int cholespy_solver_solve_f64(cholespy_gpu_solver *solver, int n_rhs, const double *b,
                               double *x, void *stream);
 
+/* x := A^{-1} b like cholespy_solver_solve_f32/f64 (the handle's precision:
+ * b and x hold floats for a float_bytes = 4 handle, doubles for 8), with the
+ * same kernels and the same results, bit for bit, but no device allocation
+ * of any kind: the work vectors (y, z, gathered right-hand sides, partial
+ * sums, split results) live in one fixed 64 MiB array in the library's
+ * device code object, one copy per GPU, held until the process ends. No
+ * counterpart in the reference; it is for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc). Block schedule only, single-system handles only. Returns
+ * CHOLESPY_ERR_INVALID_ARGUMENT when the system needs more than the array
+ * holds (3 n n_rhs + (partial slots + split rows) n_rhs elements, each
+ * region rounded up to 256 bytes; icosphere(7) fits up to n_rhs = 12 in
+ * FP64). Every call on one GPU shares that GPU's array and finds the last
+ * call's contents there: a call whose handle or n_rhs differs from the last
+ * one on the device first refills the array with fill kernels on `stream`.
+ * Calls that overlap on one GPU (concurrent streams, threads or processes
+ * sharing the library) race on the array and must be serialised by the
+ * caller. */
+int cholespy_solver_solve_noalloc(cholespy_gpu_solver *solver, int n_rhs, const void *b, void *x,
+                                  void *stream);
+
 /* One triangular pass of a solve: pass 0 = forward (reads b, fills the
  * handle's intermediate y), pass 1 = backward (reads y, writes x). Running
  * pass 0 then pass 1 with the same n_rhs equals one solve. Replaces
