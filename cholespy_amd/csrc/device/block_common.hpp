@@ -44,6 +44,13 @@ struct BlockPassArgs {
     // Optional task timeline (nullptr = off): per task, 4 words =
     // {start, after the gather phase, end, workgroup id} in wall-clock ticks (100 MHz).
     unsigned long long *trace;
+    // Leaf phase of the launch (narrow kernels; n_leaf = 0: none): n_leaf
+    // leaf task records at leaf_task, taken per wavefront from the queues
+    // lqa, before the block tasks (forward) or after them (backward).
+    const int *leaf_task;
+    int n_leaf;
+    QueueArgs lqa;
+    unsigned long long *leaf_trace;
 };
 
 // Right-hand-side layout of one task's system.
@@ -81,6 +88,13 @@ __device__ __forceinline__ VecMap vec_map(const BlockPassArgs<F> &a, const Task 
 }
 
 // The whole record is one 64-byte line: four 16-byte loads in flight together.
+__device__ __forceinline__ Task load_task_at(const int *tasks, int tk) {
+    const int4 *t = reinterpret_cast<const int4 *>(tasks + (size_t)kDevTaskInts * tk);
+    const int4 w0 = t[0], w1 = t[1], w2 = t[2], w3 = t[3];
+    return Task{w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w,
+                w2.x, w2.y, w2.z, w2.w,
+                (int64_t)(((uint64_t)(uint32_t)w3.y << 32) | (uint32_t)w3.x), w3.z};
+}
 template <typename F>
 __device__ __forceinline__ Task load_task(const BlockPassArgs<F> &a, int tk) {
     const int4 *t = reinterpret_cast<const int4 *>(a.task + (size_t)kDevTaskInts * tk);
@@ -136,6 +150,38 @@ __device__ __forceinline__ void combine_splits(const BlockPassArgs<F> &a, F *sp,
 #pragma unroll
         for (int s = 0; s < kMaxSplit; ++s)
             if (s < nsplit) p0[s * stride] = sent;
+        out(i, t, v);
+    }
+}
+
+// Combine by the last split itself, whose own results (never published) are
+// in LDS, own[i * KT + t] for row i, column t: the other nsplit - 1
+// splits' results are waited on and added in split order, then the own
+// value, so the sum is the same as combine_splits gives; the values read
+// are reset to the sentinel. Saves the last split the round trip of
+// publishing its own results and reading them back. out(i, t, value)
+// finishes row i, column t.
+template <typename F, int KT, typename Out>
+__device__ __forceinline__ void combine_splits_own(const BlockPassArgs<F> &a, F *sp, int w, int k, int sbase,
+                                                   int nsplit, const F *own, Out out) {
+    using U = typename Bits<F>::U;
+    const int total = w * k;
+    for (int idx = threadIdx.x; idx < total; idx += kTaskThreads) {
+        const int i = idx / k, t = idx - i * k;
+        F *p0 = sp + (long long)(sbase + i) * k + t;
+        const long long stride = (long long)w * k;
+        U u[kMaxSplit - 1];
+#pragma unroll
+        for (int s = 0; s < kMaxSplit - 1; ++s) u[s] = s < nsplit - 1 ? peek_bits(p0 + s * stride) : U(0);
+        F v = F(0);
+#pragma unroll
+        for (int s = 0; s < kMaxSplit - 1; ++s)
+            if (s < nsplit - 1) v += ready_value(p0 + s * stride, u[s], a.status, a.qa.spin_limit);
+        v += own[i * KT + t];
+        const F sent = from_bits<F>(Bits<F>::sentinel);
+#pragma unroll
+        for (int s = 0; s < kMaxSplit - 1; ++s)
+            if (s < nsplit - 1) p0[s * stride] = sent;
         out(i, t, v);
     }
 }

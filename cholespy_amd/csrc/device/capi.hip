@@ -35,6 +35,10 @@ static __device__ __align__(256) unsigned char scratch[kScratchBytes];
 
 namespace {
 
+// Ticket counters of a handle: six queue sets of kQueueWords (analysis,
+// row passes, block passes) and two leaf-phase sets of kLeafQueueWords.
+constexpr size_t kAllQueueWords = 6 * kQueueWords + 2 * kLeafQueueWords;
+
 thread_local std::string g_error;
 bool g_initialised = false;
 int g_device = -1;
@@ -175,7 +179,12 @@ struct cholespy_gpu_solver {
     // Ticket queues per kernel: [0..1] analysis upper/lower, [2..3] row pass
     // backward/forward, [4..5] block pass backward/forward.
     unsigned int *qctr = nullptr;
-    QueueState queues[6];
+    // [6..7] leaf phases of the fused block passes backward/forward (up to
+    // kMaxLeafQueues queues each).
+    QueueState queues[8];
+    // Leaf tasks inside the block launches instead of launches of their own:
+    // bit 0 forward, bit 1 backward (CHOLESPY_FUSED_LEAVES overrides).
+    int fused_leaves = 3;
     unsigned long long *trace[2] = {nullptr, nullptr};   // block pass timelines [backward, forward]
     std::vector<int> plan_copy[8];        // host copies of the plan for cholespy_solver_plan_arrays
     // Batches of independent systems (cholespy_batch_*): rows are
@@ -386,7 +395,7 @@ int report_give_up(cholespy_gpu_solver *s) {
         std::lock_guard<std::mutex> lock(g_scratch_mu);
         for (uint64_t &l : g_scratch_layout) l = 0;
     }
-    HIP_TRY(hipMemset(s->qctr, 0, 6 * kQueueWords * sizeof(unsigned int)));
+    HIP_TRY(hipMemset(s->qctr, 0, kAllQueueWords * sizeof(unsigned int)));
     int rc = refill_work_buffers(s, nullptr);
     if (rc != CHOLESPY_OK) return rc;
     HIP_TRY(hipDeviceSynchronize());
@@ -458,15 +467,20 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
     }
     if (e == hipSuccess) e = dev_alloc(&s->perm, (size_t)n);
     if (e == hipSuccess) e = dev_alloc(&s->lvl1, (size_t)n);
-    const size_t qwords = 6 * kQueueWords;
-    if (e == hipSuccess) e = dev_alloc(&s->qctr, qwords);
-    if (e == hipSuccess) e = hipMemset(s->qctr, 0, qwords * sizeof(unsigned int));
-    if (e == hipSuccess)
+    if (e == hipSuccess) e = dev_alloc(&s->qctr, kAllQueueWords);
+    if (e == hipSuccess) e = hipMemset(s->qctr, 0, kAllQueueWords * sizeof(unsigned int));
+    if (e == hipSuccess) {
         for (int i = 0; i < 6; ++i) s->queues[i].ctr = s->qctr + (size_t)i * kQueueWords;
-        // Block-pass ticket queues (tuning sweeps: CHOLESPY_BLOCK_QUEUES, 1..16).
-        if (const char *e = std::getenv("CHOLESPY_BLOCK_QUEUES"))
-            s->queues[4].nq = s->queues[5].nq = std::min(16, std::max(1, std::atoi(e)));
-    if (const char *e = std::getenv("CHOLESPY_LEAF_KERNELS")) s->leaf_kernels = std::atoi(e) != 0;
+        for (int i = 0; i < 2; ++i) {
+            s->queues[6 + i].ctr = s->qctr + 6 * kQueueWords + (size_t)i * kLeafQueueWords;
+            s->queues[6 + i].nq = kMaxLeafQueues;
+        }
+    }
+    // Block-pass ticket queues (tuning sweeps: CHOLESPY_BLOCK_QUEUES, 1..16).
+    if (const char *v = std::getenv("CHOLESPY_BLOCK_QUEUES"))
+        s->queues[4].nq = s->queues[5].nq = std::min(16, std::max(1, std::atoi(v)));
+    if (const char *v = std::getenv("CHOLESPY_LEAF_KERNELS")) s->leaf_kernels = std::atoi(v) != 0;
+    if (const char *v = std::getenv("CHOLESPY_FUSED_LEAVES")) s->fused_leaves = std::atoi(v) & 3;
     if (e != hipSuccess) {
         cholespy_solver_destroy(s);
         return hip_fail(e, "cholespy_solver_create");
@@ -872,6 +886,8 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
             if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for the block solve");
             grid = std::max(1, std::min(cap, bf.ntasks));
         }
+        bf.fused_leaves = (s->fused_leaves >> (forward ? 0 : 1)) & 1;
+        bf.leaf_queues = &s->queues[6 + (forward ? 1 : 0)];
         HIP_TRY(launch_block_pass(fbytes, forward, bf, grid, s->queues[4 + (forward ? 1 : 0)], st));
     }
     return CHOLESPY_OK;

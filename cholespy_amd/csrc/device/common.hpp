@@ -160,6 +160,7 @@ __device__ __forceinline__ int wait_nonzero(const int *p, int *status, uint32_t 
 // zeroed: the last workgroup to finish resets them, so launches need no
 // host-side state and can be replayed from a captured graph.
 constexpr int kMaxQueues = 16;
+constexpr int kMaxLeafQueues = 64;   // queues of a fused launch's leaf phase
 constexpr int kQueueStride = 16;   // 128 bytes between counters
 
 struct QueueArgs {
@@ -184,12 +185,18 @@ __device__ __forceinline__ int take_ticket(const QueueArgs &qa, int q, int n) {
 // before the next launch. (An acquire-release here would write back and
 // invalidate the workgroup's L2 at every exit, under the tasks still
 // running.)
-__device__ __forceinline__ void finish_queues(const QueueArgs &qa) {
+// `extra` (extra_nq > 0): a second set of extra_nq queues the launch used
+// (a fused launch's leaf queues, counters kQueueStride apart), zeroed with
+// them; every wavefront of the workgroup must be done taking tickets.
+__device__ __forceinline__ void finish_queues(const QueueArgs &qa, unsigned int *extra = nullptr,
+                                              int extra_nq = 0) {
     unsigned int *done = qa.ctr + (size_t)kMaxQueues * kQueueStride;
     const unsigned int d = __hip_atomic_fetch_add(done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (d + 1 == gridDim.x * gridDim.y * gridDim.z) {
         for (int q = 0; q < qa.nq; ++q)
             __hip_atomic_store(qa.ctr + (size_t)q * kQueueStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < extra_nq; ++q)
+            __hip_atomic_store(extra + (size_t)q * kQueueStride, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }

@@ -123,7 +123,12 @@ struct BlockPassBuffers {
     // Batches of independent systems (nullptr for a single system).
     const int *blk_sys = nullptr, *sys_k = nullptr;
     const long long *sys_vbase = nullptr, *sys_pbase = nullptr, *sys_sbase = nullptr;
+    // Leaf tasks inside the block launch (its leaf phase, ticket queues
+    // leaf_queues with up to 64 queues) instead of a launch of their own.
+    bool fused_leaves = false;
+    QueueState *leaf_queues = nullptr;
 };
+constexpr size_t kLeafQueueWords = 64 * 16 + 16;   // leaf-phase queue counters (padded)
 // Dynamic LDS of a block pass: one chunk of the vector it stages (r_B of
 // up to need_rows = max m forward, [y_B; z_R] of up to max m + |R| backward),
 // at least 256 rows for the per-thread sums that reuse the space.

@@ -47,6 +47,225 @@ constexpr int kMU = CHOLESPY_KMU;   // M loads issued together per thread
 // may wait on other tasks: their latency overlaps the wait and the gather.
 constexpr int kPre = CHOLESPY_KPRE;
 
+// Leaf blocks (block_plan.hpp: no child blocks, at most kLeafMaxM rows and
+// kLeafMaxRows rows of M_B): one wavefront per task. Their forward tasks
+// take nothing from other blocks (r_B = (P b)_B) and their backward tasks
+// give nothing to other blocks, so they are the first forward tickets and
+// the last backward tickets. They run either inside the block pass's launch
+// (its leaf phase: every wavefront takes leaf tickets from queues of its
+// own before (forward) or after (backward) the workgroup's block tasks, and
+// a backward leaf task waits on the z_R the block tasks publish) or in a
+// launch of their own before / after the block launch (no tickets and no
+// waits). The tasks and the results are those of the block kernels; only
+// the lane layout and the reduction order inside a task differ.
+constexpr int kLeafWaves = kTaskThreads / kWave;
+#ifndef CHOLESPY_KLEAFMU
+#define CHOLESPY_KLEAFMU 16
+#endif
+constexpr int kLeafMU = CHOLESPY_KLEAFMU;   // M loads in flight per lane
+
+// Orders this wavefront's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Forward leaf task tk of the records `tasks` (r: kLeafMaxM x KT of this
+// wavefront's LDS).
+template <typename F, int KT>
+__device__ __forceinline__ void leaf_forward_task(const BlockPassArgs<F> &a, const int *tasks,
+                                                  unsigned long long *trace, int tk, F *r) {
+    const int lane = threadIdx.x % kWave;
+    if (trace && lane == 0) trace[4 * (size_t)tk] = wall_clock64();
+    const Task task = load_task_at(tasks, tk);
+    const int first = task.first, last = task.last, a0 = task.a0, m = task.m;
+    const int rows = m + task.R;
+    const F *M = a.M + task.moff;
+    const VecMap vm = vec_map(a, task);
+    const int k = vm.k;
+    // Lane = row rho of M_B; row rho uses columns [0, cend).
+    const int rho = first + lane;
+    const int cend = rho < last ? min(rho + 1, m) : 0;
+    F mv[kLeafMU];
+    int64_t o = fwd_off(rows, 0, rho);
+#pragma unroll
+    for (int j = 0; j < kLeafMU; ++j) {
+        mv[j] = j < cend ? M[o] : F(0);
+        o += rows - j - 1;
+    }
+    // r_B = (P b)_B, rows [0, m).
+    if (lane < m) {
+        const long long pr = vm.vbase + (long long)a.perm[a0 + lane] * k;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) r[lane * KT + t] = t < k ? a.b[pr + t] : F(0);
+    }
+    wave_sync();
+    if (trace && lane == 0) trace[4 * (size_t)tk + 1] = wall_clock64();
+    F acc[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] = F(0);
+    for (int c = 0;;) {
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            if (c + j < cend) {
+                const F *rr = r + (c + j) * KT;
+#pragma unroll
+                for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], rr[t], acc[t]);
+            }
+        }
+        c += kLeafMU;
+        if (c >= cend) break;
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            mv[j] = c + j < cend ? M[o] : F(0);
+            o += rows - (c + j) - 1;
+        }
+    }
+    if (rho < last) {
+        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            if (t >= k) break;
+            if (rho < m) {
+                const long long q = (long long)(a0 + rho) * k + t;
+                yv[q] = acc[t];
+                zv[q] = from_bits<F>(Bits<F>::sentinel);
+            } else {
+                publish(&pv[(long long)(task.r0 + rho - m) * k + t], acc[t]);
+            }
+        }
+    }
+    if (trace && lane == 0) {
+        trace[4 * (size_t)tk + 2] = wall_clock64();
+        trace[4 * (size_t)tk + 3] = blockIdx.x;
+    }
+    // The next task reuses r.
+    wave_sync();
+}
+
+// Backward leaf task tk (v: kLeafMaxRows x KT of this wavefront's LDS).
+// WAIT: z_R is published by block tasks of the same launch (read with the
+// bounded wait); otherwise it is complete before the launch.
+template <typename F, int KT, bool WAIT>
+__device__ __forceinline__ void leaf_backward_task(const BlockPassArgs<F> &a, const int *tasks,
+                                                   unsigned long long *trace, int tk, F *v) {
+    using U = typename Bits<F>::U;
+    const int lane = threadIdx.x % kWave;
+    if (trace && lane == 0) trace[4 * (size_t)tk] = wall_clock64();
+    const Task task = load_task_at(tasks, tk);
+    const int first = task.first, last = task.last, a0 = task.a0, m = task.m, r0 = task.r0, R = task.R;
+    const int rows = m + R;
+    const F *M = a.M + task.moff;
+    const VecMap vm = vec_map(a, task);
+    const int k = vm.k;
+    F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
+    // Lanes l and l + 32 share column c = first + l % 32 of M_B and take the
+    // rows rho = first + l / 32 + 2 j, rho >= c (rows of one step are
+    // consecutive entries of the row-major packed M_B across the lanes).
+    const int c = first + (lane & 31), h = lane >> 5;
+    const bool active = c < last;
+    F mv[kLeafMU];
+#pragma unroll
+    for (int j = 0; j < kLeafMU; ++j) {
+        const int rr = first + h + 2 * j;
+        mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
+    }
+    // v = [y_B ; z_R], rows [first, rows): y from the forward pass, z_R from
+    // the backward block tasks.
+    for (int i = first + lane; i < rows; i += kWave) {
+        if (i < m || !WAIT) {
+            const F *src = i < m ? yv + (long long)(a0 + i) * k : zv + (long long)a.rlist[r0 + i - m] * k;
+#pragma unroll
+            for (int t = 0; t < KT; ++t) v[(i - first) * KT + t] = t < k ? src[t] : F(0);
+        } else {
+            const F *src = zv + (long long)a.rlist[r0 + i - m] * k;
+            U u[KT];
+#pragma unroll
+            for (int t = 0; t < KT; ++t) u[t] = t < k ? peek_bits(src + t) : U(0);
+#pragma unroll
+            for (int t = 0; t < KT; ++t)
+                v[(i - first) * KT + t] = t < k ? ready_value(src + t, u[t], a.status, a.qa.spin_limit) : F(0);
+        }
+    }
+    wave_sync();
+    if (trace && lane == 0) trace[4 * (size_t)tk + 1] = wall_clock64();
+    F acc[KT];
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] = F(0);
+    for (int j0 = 0;; j0 += kLeafMU) {
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            const int rr = first + h + 2 * (j0 + j);
+            if (active && rr >= c && rr < rows) {
+                const F *vv = v + (rr - first) * KT;
+#pragma unroll
+                for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], vv[t], acc[t]);
+            }
+        }
+        if (first + h + 2 * (j0 + kLeafMU) >= rows) break;
+#pragma unroll
+        for (int j = 0; j < kLeafMU; ++j) {
+            const int rr = first + h + 2 * (j0 + kLeafMU + j);
+            mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < KT; ++t) acc[t] += __shfl_xor(acc[t], 32, kWave);
+    if (h == 0 && active) {
+#pragma unroll
+        for (int t = 0; t < KT; ++t) {
+            if (t >= k) break;
+            publish(&zv[(long long)(a0 + c) * k + t], acc[t]);
+            const long long q = vm.vbase + (long long)a.perm[a0 + c] * k + t;
+            if (a.xmode == 0) {
+                static_cast<F *>(a.x)[q] = acc[t];
+            } else {
+                double *x64 = static_cast<double *>(a.x);
+                x64[q] = (a.xmode == 2 ? x64[q] : 0.0) + (double)acc[t];
+            }
+        }
+    }
+    // Restore the sentinel in the block's partial slots for the next solve
+    // (once per block: its first column task). Its r_B rows in rbuf need no
+    // reset: a leaf block has m < split_gather, so no forward task of the
+    // narrow list (the only one the leaf tasks serve) gathers it into rbuf,
+    // and a wide solve's backward pass resets what its forward pass wrote.
+    if (first == 0) {
+        const F sent = from_bits<F>(Bits<F>::sentinel);
+        for (int idx = lane; idx < R * k; idx += kWave) pv[(long long)r0 * k + idx] = sent;
+    }
+    if (trace && lane == 0) {
+        trace[4 * (size_t)tk + 2] = wall_clock64();
+        trace[4 * (size_t)tk + 3] = blockIdx.x;
+    }
+    // The next task reuses v.
+    wave_sync();
+}
+
+// Leaf phase of a fused block launch: every wavefront takes leaf tickets
+// (a.lqa, nq queues; wavefront w serves queue w mod nq) until its queue is
+// exhausted. The next ticket is requested while a task runs: a held leaf
+// ticket is run by this running wavefront right after, so no wait can end
+// on a ticket nobody runs. lds: this workgroup's dynamic LDS (kLeafWaves
+// slices of kLeafMaxRows x KT).
+template <typename F, int KT, bool FORWARD>
+__device__ __forceinline__ void leaf_phase(const BlockPassArgs<F> &a, F *lds) {
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    F *mine = lds + (size_t)wave * (kLeafMaxRows * KT);
+    const int q = (blockIdx.x * kLeafWaves + wave) % a.lqa.nq;
+    int t = lane == 0 ? take_ticket(a.lqa, q, a.n_leaf) : 0;
+    t = __shfl(t, 0, kWave);
+    while (t >= 0) {
+        int next = lane == 0 ? take_ticket(a.lqa, q, a.n_leaf) : 0;
+        if (FORWARD)
+            leaf_forward_task<F, KT>(a, a.leaf_task, a.leaf_trace, t, mine);
+        else
+            leaf_backward_task<F, KT, true>(a, a.leaf_task, a.leaf_trace, t, mine);
+        t = __shfl(next, 0, kWave);
+    }
+}
+
 template <typename F, int KT>
 __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -56,6 +275,10 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
     __shared__ int s_next;
     const int tid = threadIdx.x;
     const int qid = blockIdx.x % a.qa.nq;
+    if (a.n_leaf > 0) {   // leaf tasks first: they wait on nothing
+        leaf_phase<F, KT, true>(a, rv);
+        __syncthreads();
+    }
     int tk = wg_first_ticket(a.qa, qid, a.ntasks, &s_next);
     while (tk >= 0) {
         if (a.trace && tid == 0) a.trace[4 * (size_t)tk] = wall_clock64();
@@ -89,6 +312,10 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         const int clo = kind == 3 ? task.lo : 0, chi = kind == 3 ? task.hi : m;
         const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
         const int srow = task.sbase + sidx * (last - first) - first;
+        // The last split of a group keeps its own results for the combine, in
+        // LDS at [(rho - first) * KT + t]: the one entry of the products'
+        // sums that only this row's reduction reads, after it has read it.
+        const bool keep_own = kind == 3 && sidx == nsp - 1 && k <= KT;
         // This thread's columns of the first chunk (see the product below);
         // the first kPre of them are loaded before the gather.
         auto cols_of = [&](int c0, int c1, int &cb, int &ce) {
@@ -184,7 +411,9 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
                     F s = rv[lr * KT + t];
                     for (int g = 1; g < segs; ++g) s += rv[(g * lanes + lr) * KT + t];
                     const int col = t0 + t;
-                    if (kind == 3)
+                    if (keep_own)
+                        rv[lr * KT + t] = s;
+                    else if (kind == 3)
                         publish(&sp[(long long)(srow + rho) * k + col], s);
                     else
                         finish(rho, col, s);
@@ -192,9 +421,13 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
             }
             __syncthreads();
         }
-        if (kind == 3 && combines_splits(sidx, nsp))
+        if (keep_own) {   // (k <= KT: one tile, its closing barrier orders the LDS)
+            combine_splits_own<F, KT>(a, sp, last - first, k, task.sbase, nsp, rv,
+                                      [&](int i, int t, F v) { finish(first + i, t, v); });
+        } else if (kind == 3 && combines_splits(sidx, nsp)) {
             combine_splits(a, sp, last - first, k, task.sbase, nsp,
                            [&](int i, int t, F v) { finish(first + i, t, v); });
+        }
         if (a.trace && tid == 0) {
             a.trace[4 * (size_t)tk + 2] = wall_clock64();
             a.trace[4 * (size_t)tk + 3] = blockIdx.x;
@@ -204,7 +437,8 @@ __global__ __launch_bounds__(kTaskThreads) void block_forward(BlockPassArgs<F> a
         tk = s_next;
         __syncthreads();
     }
-    if (tid == 0) finish_queues(a.qa);
+    __syncthreads();
+    if (tid == 0) finish_queues(a.qa, a.lqa.ctr, a.n_leaf > 0 ? a.lqa.nq : 0);
 }
 
 template <typename F, int KT>
@@ -249,6 +483,9 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         const int hlo = kind == 1 ? task.lo : 0, hhi = kind == 1 ? task.hi : rows;
         const int nsp = split_count(task.nsplit), sidx = split_index(task.nsplit);
         const int srow = task.sbase + sidx * (last - first) - first;
+        // The last split of a group keeps its own results for the combine, in
+        // LDS at [(c - first) * KT + t] (see block_forward).
+        const bool keep_own = kind == 1 && sidx == nsp - 1 && k <= KT;
         // This thread's rows of a chunk (see the product below); the first
         // kPre of the first chunk are loaded before the gather.
         auto rows_of = [&](int h0, int h1, int &rb, int &re) {
@@ -336,7 +573,9 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
                     F s = vs[lr * KT + t];
                     for (int g = 1; g < segs; ++g) s += vs[(g * lanes + lr) * KT + t];
                     const int col = t0 + t;
-                    if (kind == 1)
+                    if (keep_own)
+                        vs[lr * KT + t] = s;
+                    else if (kind == 1)
                         publish(&sp[(long long)(srow + c) * k + col], s);
                     else
                         finish(c, col, s);
@@ -344,9 +583,13 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
             }
             __syncthreads();
         }
-        if (kind == 1 && combines_splits(sidx, nsp))
+        if (keep_own) {   // (k <= KT: one tile, its closing barrier orders the LDS)
+            combine_splits_own<F, KT>(a, sp, last - first, k, task.sbase, nsp, vs,
+                                      [&](int i, int t, F v) { finish(first + i, t, v); });
+        } else if (kind == 1 && combines_splits(sidx, nsp)) {
             combine_splits(a, sp, last - first, k, task.sbase, nsp,
                            [&](int i, int t, F v) { finish(first + i, t, v); });
+        }
         // Restore the sentinel in this block's partial slots for the next solve
         // (once per block: its first column task, or that task's first split).
         if (first == 0 && sidx == 0) {
@@ -364,185 +607,31 @@ __global__ __launch_bounds__(kTaskThreads) void block_backward(BlockPassArgs<F> 
         tk = s_next;
         __syncthreads();
     }
-    if (tid == 0) finish_queues(a.qa);
+    if (a.n_leaf > 0) {   // leaf tasks last: they wait on the block tasks' z
+        leaf_phase<F, KT, false>(a, vs);
+        __syncthreads();
+    }
+    if (tid == 0) finish_queues(a.qa, a.lqa.ctr, a.n_leaf > 0 ? a.lqa.nq : 0);
 }
 
-// Leaf blocks (block_plan.hpp: no child blocks, at most kLeafMaxM rows and
-// kLeafMaxRows rows of M_B): one wavefront per task, kLeafWaves tasks per
-// workgroup, no tickets and no waits. The forward launch runs before the
-// forward block pass (leaf rows receive no partials, so r_B = (P b)_B); the
-// backward launch runs after the backward block pass, when every z_R is
-// final. The tasks and the results are those of the block kernels; only the
-// lane layout and the reduction order inside a task differ.
-constexpr int kLeafWaves = kTaskThreads / kWave;
-#ifndef CHOLESPY_KLEAFMU
-#define CHOLESPY_KLEAFMU 16
-#endif
-constexpr int kLeafMU = CHOLESPY_KLEAFMU;   // M loads in flight per lane
-
-// Orders this wavefront's LDS writes before its later LDS reads by other lanes.
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
+// Leaf tasks in a launch of their own: one wavefront per task, a.ntasks
+// tasks at a.task.
 template <typename F, int KT>
 __global__ __launch_bounds__(kTaskThreads) void leaf_forward(BlockPassArgs<F> a) {
     __shared__ F rs[kLeafWaves][kLeafMaxM * KT];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
     const int tk = blockIdx.x * kLeafWaves + wave;
     if (tk >= a.ntasks) return;
-    if (a.trace && lane == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-    const Task task = load_task(a, tk);
-    const int first = task.first, last = task.last, a0 = task.a0, m = task.m;
-    const int rows = m + task.R;
-    const F *M = a.M + task.moff;
-    const VecMap vm = vec_map(a, task);
-    const int k = vm.k;
-    // Lane = row rho of M_B; row rho uses columns [0, cend).
-    const int rho = first + lane;
-    const int cend = rho < last ? min(rho + 1, m) : 0;
-    F mv[kLeafMU];
-    int64_t o = fwd_off(rows, 0, rho);
-#pragma unroll
-    for (int j = 0; j < kLeafMU; ++j) {
-        mv[j] = j < cend ? M[o] : F(0);
-        o += rows - j - 1;
-    }
-    // r_B = (P b)_B, rows [0, m).
-    F *r = rs[wave];
-    if (lane < m) {
-        const long long pr = vm.vbase + (long long)a.perm[a0 + lane] * k;
-#pragma unroll
-        for (int t = 0; t < KT; ++t) r[lane * KT + t] = t < k ? a.b[pr + t] : F(0);
-    }
-    wave_sync();
-    if (a.trace && lane == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-    F acc[KT];
-#pragma unroll
-    for (int t = 0; t < KT; ++t) acc[t] = F(0);
-    for (int c = 0;;) {
-#pragma unroll
-        for (int j = 0; j < kLeafMU; ++j) {
-            if (c + j < cend) {
-                const F *rr = r + (c + j) * KT;
-#pragma unroll
-                for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], rr[t], acc[t]);
-            }
-        }
-        c += kLeafMU;
-        if (c >= cend) break;
-#pragma unroll
-        for (int j = 0; j < kLeafMU; ++j) {
-            mv[j] = c + j < cend ? M[o] : F(0);
-            o += rows - (c + j) - 1;
-        }
-    }
-    if (rho < last) {
-        F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            if (t >= k) break;
-            if (rho < m) {
-                const long long q = (long long)(a0 + rho) * k + t;
-                yv[q] = acc[t];
-                zv[q] = from_bits<F>(Bits<F>::sentinel);
-            } else {
-                publish(&pv[(long long)(task.r0 + rho - m) * k + t], acc[t]);
-            }
-        }
-    }
-    if (a.trace && lane == 0) {
-        a.trace[4 * (size_t)tk + 2] = wall_clock64();
-        a.trace[4 * (size_t)tk + 3] = blockIdx.x;
-    }
+    leaf_forward_task<F, KT>(a, a.task, a.trace, tk, rs[wave]);
 }
 
 template <typename F, int KT>
 __global__ __launch_bounds__(kTaskThreads) void leaf_backward(BlockPassArgs<F> a) {
     __shared__ F vsh[kLeafWaves][kLeafMaxRows * KT];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int wave = threadIdx.x / kWave;
     const int tk = blockIdx.x * kLeafWaves + wave;
     if (tk >= a.ntasks) return;
-    if (a.trace && lane == 0) a.trace[4 * (size_t)tk] = wall_clock64();
-    const Task task = load_task(a, tk);
-    const int first = task.first, last = task.last, a0 = task.a0, m = task.m, r0 = task.r0, R = task.R;
-    const int rows = m + R;
-    const F *M = a.M + task.moff;
-    const VecMap vm = vec_map(a, task);
-    const int k = vm.k;
-    F *yv = a.y + vm.vbase, *zv = a.z + vm.vbase, *pv = a.part + vm.pbase;
-    // Lanes l and l + 32 share column c = first + l % 32 of M_B and take the
-    // rows rho = first + l / 32 + 2 j, rho >= c (rows of one step are
-    // consecutive entries of the row-major packed M_B across the lanes).
-    const int c = first + (lane & 31), h = lane >> 5;
-    const bool active = c < last;
-    F mv[kLeafMU];
-#pragma unroll
-    for (int j = 0; j < kLeafMU; ++j) {
-        const int rr = first + h + 2 * j;
-        mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
-    }
-    // v = [y_B ; z_R], rows [first, rows): y from the forward pass, z_R from
-    // the backward block pass (both complete before this launch).
-    F *v = vsh[wave];
-    for (int i = first + lane; i < rows; i += kWave) {
-        const F *src = i < m ? yv + (long long)(a0 + i) * k : zv + (long long)a.rlist[r0 + i - m] * k;
-#pragma unroll
-        for (int t = 0; t < KT; ++t) v[(i - first) * KT + t] = t < k ? src[t] : F(0);
-    }
-    wave_sync();
-    if (a.trace && lane == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-    F acc[KT];
-#pragma unroll
-    for (int t = 0; t < KT; ++t) acc[t] = F(0);
-    for (int j0 = 0;; j0 += kLeafMU) {
-#pragma unroll
-        for (int j = 0; j < kLeafMU; ++j) {
-            const int rr = first + h + 2 * (j0 + j);
-            if (active && rr >= c && rr < rows) {
-                const F *vv = v + (rr - first) * KT;
-#pragma unroll
-                for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], vv[t], acc[t]);
-            }
-        }
-        if (first + h + 2 * (j0 + kLeafMU) >= rows) break;
-#pragma unroll
-        for (int j = 0; j < kLeafMU; ++j) {
-            const int rr = first + h + 2 * (j0 + kLeafMU + j);
-            mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < KT; ++t) acc[t] += __shfl_xor(acc[t], 32, kWave);
-    if (h == 0 && active) {
-#pragma unroll
-        for (int t = 0; t < KT; ++t) {
-            if (t >= k) break;
-            publish(&zv[(long long)(a0 + c) * k + t], acc[t]);
-            const long long q = vm.vbase + (long long)a.perm[a0 + c] * k + t;
-            if (a.xmode == 0) {
-                static_cast<F *>(a.x)[q] = acc[t];
-            } else {
-                double *x64 = static_cast<double *>(a.x);
-                x64[q] = (a.xmode == 2 ? x64[q] : 0.0) + (double)acc[t];
-            }
-        }
-    }
-    // Restore the sentinel in the block's partial slots for the next solve
-    // (once per block: its first column task). Its r_B rows in rbuf need no
-    // reset: a leaf block has m < split_gather, so no forward task of the
-    // narrow list (the only one these launches serve) gathers it into rbuf,
-    // and a wide solve's backward pass resets what its forward pass wrote.
-    if (first == 0) {
-        const F sent = from_bits<F>(Bits<F>::sentinel);
-        for (int idx = lane; idx < R * k; idx += kWave) pv[(long long)r0 * k + idx] = sent;
-    }
-    if (a.trace && lane == 0) {
-        a.trace[4 * (size_t)tk + 2] = wall_clock64();
-        a.trace[4 * (size_t)tk + 3] = blockIdx.x;
-    }
+    leaf_backward_task<F, KT, false>(a, a.task, a.trace, tk, vsh[wave]);
 }
 
 template <typename F>
@@ -589,15 +678,40 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.sys_pbase = bf.sys_pbase;
     a.sys_sbase = bf.sys_sbase;
     queues.fill(a.qa);
+    a.leaf_task = nullptr;
+    a.n_leaf = 0;
+    a.leaf_trace = nullptr;
+    if (bf.leaf_queues) bf.leaf_queues->fill(a.lqa);
+    else a.lqa = QueueArgs{nullptr, 1, a.qa.spin_limit};
     if (is_panel_tile(tile)) {
         if constexpr (sizeof(F) == 4) return launch_panel_f32(forward, a, tile / 16, grid, stream);
         else return launch_panel_f64(forward, a, tile / 16, grid, stream);
     }
-    const size_t lds = block_pass_lds(sizeof(F), forward ? bf.max_m : bf.max_rows, tile);
-    // Leaf tasks (forward: the first n_leaf, backward: the last n_leaf) in
-    // their own launch, before / after the block pass over the others.
+    size_t lds = block_pass_lds(sizeof(F), forward ? bf.max_m : bf.max_rows, tile);
+    // Leaf tasks (forward: the first n_leaf, backward: the last n_leaf): in
+    // the block launch's leaf phase (fused), or in their own launch before /
+    // after the block pass over the others.
     const int n_leaf = std::min(bf.n_leaf, bf.ntasks);
     const int n_main = bf.ntasks - n_leaf;
+    if (bf.fused_leaves && n_leaf > 0) {
+        BlockPassArgs<F> f = a;
+        const size_t leaf_off = forward ? 0 : (size_t)n_main;
+        const size_t main_off = forward ? (size_t)n_leaf : 0;
+        f.task = a.task + main_off * kDevTaskInts;
+        f.ntasks = n_main;
+        f.leaf_task = a.task + leaf_off * kDevTaskInts;
+        f.n_leaf = n_leaf;
+        if (a.trace) {
+            f.trace = a.trace + 4 * main_off;
+            f.leaf_trace = a.trace + 4 * leaf_off;
+        }
+        lds = std::max(lds, (size_t)kLeafWaves * kLeafMaxRows * kNarrowTile * sizeof(F));
+        const int g = std::max(1, std::min(grid, std::max(n_main, (n_leaf + kLeafWaves - 1) / kLeafWaves)));
+        // Every leaf queue needs a wavefront serving it (wavefront w serves
+        // queue w mod nq, and the first workgroups are dispatched first).
+        f.lqa.nq = std::max(1, std::min(f.lqa.nq, g * kLeafWaves));
+        return launch_narrow<F>(forward, f, tile, lds, g, stream);
+    }
     auto leaf = [&](int offset) -> hipError_t {
         BlockPassArgs<F> l = a;
         l.task = a.task + (size_t)offset * kDevTaskInts;
