@@ -17,6 +17,7 @@
 #include <atomic>
 #include <mutex>
 #include <thread>
+#include <sys/stat.h>
 #include <vector>
 
 #include "../../../include/cholespy_amd.h"
@@ -123,7 +124,7 @@ struct DeviceBlockPlan {
 
 struct cholespy_gpu_solver;
 namespace {
-BlockPlanOptions plan_options();
+BlockPlanOptions plan_options(const BlockPlanOptions &base = BlockPlanOptions());
 int upload_plan(cholespy_gpu_solver *s, const BlockPlan &P, const std::vector<int> *blk_sys = nullptr);
 }  // namespace
 
@@ -185,6 +186,12 @@ struct cholespy_gpu_solver {
     // Leaf tasks inside the block launches instead of launches of their own:
     // bit 0 forward, bit 1 backward (CHOLESPY_FUSED_LEAVES overrides).
     int fused_leaves = 3;
+    // Plan options the handle plans with (before environment overrides),
+    // and a tile width per right-hand-side class (0 = the rule of
+    // block_tile): both set by cholespy_solver_autotune.
+    BlockPlanOptions plan_base;
+    int tile_for_class[4] = {0, 0, 0, 0};
+    std::string tune_report;
     unsigned long long *trace[2] = {nullptr, nullptr};   // block pass timelines [backward, forward]
     std::vector<int> plan_copy[8];        // host copies of the plan for cholespy_solver_plan_arrays
     // Batches of independent systems (cholespy_batch_*): rows are
@@ -210,10 +217,10 @@ struct cholespy_gpu_solver {
 };
 
 namespace {
-// Block-size limits; CHOLESPY_MAX_BLOCK and CHOLESPY_LEAF_MAX override
-// them (tuning sweeps).
-BlockPlanOptions plan_options() {
-    BlockPlanOptions opt;
+// Plan options: `base` (the defaults, or what cholespy_solver_autotune
+// chose) with the CHOLESPY_* environment knobs as overrides on top.
+BlockPlanOptions plan_options(const BlockPlanOptions &base) {
+    BlockPlanOptions opt = base;
     if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
     // Task widths are limited to what the block kernels' lane layouts cover.
@@ -600,7 +607,7 @@ int cholespy_solver_upload_factor(cholespy_gpu_solver *s, const int *perm, const
     if (rc != CHOLESPY_OK) return rc;
     s->analysed[0] = s->analysed[1] = false;
 
-    BlockPlanOptions opt = plan_options();
+    BlockPlanOptions opt = plan_options(s->plan_base);
     BlockPlan P;
     try {
         P = build_block_plan(n, colptr, rowidx, val, opt);
@@ -654,7 +661,7 @@ int cholespy_solver_refactor(cholespy_gpu_solver *s, const double *val, int64_t 
             return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "factor diagonal must be positive and finite");
     }
     BlockPlan &P = s->host_plan;
-    fill_block_plan_values(P, s->host_colptr.data(), s->host_rowidx.data(), val, plan_options().threads);
+    fill_block_plan_values(P, s->host_colptr.data(), s->host_rowidx.data(), val, plan_options(s->plan_base).threads);
     // Solves still in flight read the old values: wait for them before
     // overwriting anything.
     HIP_TRY(hipDeviceSynchronize());
@@ -821,6 +828,17 @@ int solve_rows(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStream_
     return CHOLESPY_OK;
 }
 
+// Tile width of a solve with k right-hand sides: the handle's tuned choice
+// for k's class when it holds k (a panel tile at least k wide; the narrow
+// tile only for k <= 4), else block_tile's rule.
+int effective_tile(const cholespy_gpu_solver *s, int fbytes, int k) {
+    const int rule = block_tile(fbytes, k);
+    const int t = s->is_batch ? 0 : s->tile_for_class[tile_slot(rule)];
+    if (t == kNarrowTile && k <= kNarrowTile) return t;
+    if (t >= 32 && t >= k) return t;
+    return rule;
+}
+
 // Work buffers of one block solve (the handle's own, or the library's
 // scratch array for cholespy_solver_solve_noalloc).
 struct WorkSet {
@@ -867,7 +885,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
     for (int pass = 0; pass < 2; ++pass) {
         if (only_pass >= 0 && pass != only_pass) continue;
         const bool forward = pass == 0;
-        const int tile = block_tile(fbytes, bf.max_k);
+        const int tile = effective_tile(s, fbytes, bf.max_k);
         if (is_panel_tile(tile)) {
             bf.ntasks = forward ? d.n_fwd_tasks_wide : d.n_bwd_tasks_wide;
             bf.n_leaf = 0;
@@ -887,6 +905,7 @@ int solve_blocks(cholespy_gpu_solver *s, int k, const void *b, void *x, hipStrea
             grid = std::max(1, std::min(cap, bf.ntasks));
         }
         bf.fused_leaves = (s->fused_leaves >> (forward ? 0 : 1)) & 1;
+        bf.tile = tile;
         bf.leaf_queues = &s->queues[6 + (forward ? 1 : 0)];
         HIP_TRY(launch_block_pass(fbytes, forward, bf, grid, s->queues[4 + (forward ? 1 : 0)], st));
     }
@@ -1502,3 +1521,348 @@ int cholespy_solver_solve_noalloc(cholespy_gpu_solver *s, int n_rhs, const void 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Autotuner. Per system (keyed by the factor's pattern, the precision, the
+// right-hand-side class and the device) it times candidate configurations
+// on the device and keeps the fastest: the schedule (row-per-wavefront
+// passes or the block passes), the tile (vector-FMA or MFMA panel kernels,
+// and the panel width), the block plan's shape (chain cap, split sizes,
+// gather threshold, the wide list's level targets), the leaf phase and the
+// ticket queues. Coordinate descent from the hand-set defaults: one knob at
+// a time over its candidates, a change kept only when it is at least 1 %
+// faster, so the result is never slower than the defaults as measured.
+// Knobs set through CHOLESPY_* environment variables are not varied (they
+// stay overrides). The winner is cached in a text file, one line per key.
+namespace {
+
+struct TuneConfig {
+    int path = CHOLESPY_PATH_BLOCKS;
+    int tile = 0;            // 0 = block_tile's rule
+    int fused = 3;
+    int queues = kMaxQueues;
+    BlockPlanOptions opt;
+};
+
+uint64_t fnv1a(uint64_t h, const void *p, size_t bytes) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < bytes; ++i) {
+        h ^= c[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+std::string tune_key(const cholespy_gpu_solver *s, int k) {
+    uint64_t h = 1469598103934665603ull;
+    h = fnv1a(h, &s->n, sizeof(s->n));
+    h = fnv1a(h, s->host_colptr.data(), s->host_colptr.size() * sizeof(int));
+    h = fnv1a(h, s->host_rowidx.data(), s->host_rowidx.size() * sizeof(int));
+    hipDeviceProp_t prop;
+    std::string arch = "gpu";
+    int cus = 0;
+    if (hipGetDeviceProperties(&prop, s->device) == hipSuccess) {
+        arch = prop.gcnArchName;
+        cus = prop.multiProcessorCount;
+        const size_t colon = arch.find(':');
+        if (colon != std::string::npos) arch.resize(colon);
+    }
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%016llx-n%d-f%d-t%d-%s-cu%d", (unsigned long long)h, s->n, s->fbytes * 8,
+                  block_tile(s->fbytes, k), arch.c_str(), cus);
+    return buf;
+}
+
+// Field order of a cache line after the key.
+std::string format_config(const TuneConfig &c) {
+    char buf[256];
+    std::snprintf(buf, sizeof buf,
+                  "path=%d tile=%d fused=%d queues=%d max_block=%d split_entries=%d max_split=%d split_gather=%d "
+                  "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d",
+                  c.path, c.tile, c.fused, c.queues, c.opt.max_block, c.opt.split_entries, c.opt.max_split,
+                  c.opt.split_gather, c.opt.wide_level_tasks, c.opt.wide_min_split_entries, c.opt.wide_fwd_rows_big);
+    return buf;
+}
+
+bool parse_config(const std::string &text, TuneConfig &c) {
+    int v[11];
+    if (std::sscanf(text.c_str(),
+                    "path=%d tile=%d fused=%d queues=%d max_block=%d split_entries=%d max_split=%d split_gather=%d "
+                    "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d",
+                    &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10]) != 11)
+        return false;
+    if ((v[0] != CHOLESPY_PATH_ROWS && v[0] != CHOLESPY_PATH_BLOCKS) || v[2] < 0 || v[2] > 3 || v[3] < 1 ||
+        v[3] > kMaxQueues || v[4] < 0 || v[5] < 0 || v[6] < 1 || v[6] > 16 || v[7] < 1 || v[8] < 0 ||
+        v[9] < 1 || v[10] < 1 || v[10] > 64)
+        return false;
+    c.path = v[0];
+    c.tile = v[1];
+    c.fused = v[2];
+    c.queues = v[3];
+    c.opt.max_block = v[4];
+    c.opt.split_entries = v[5];
+    c.opt.max_split = v[6];
+    c.opt.split_gather = v[7];
+    c.opt.wide_level_tasks = v[8];
+    c.opt.wide_min_split_entries = v[9];
+    c.opt.wide_fwd_rows_big = v[10];
+    return true;
+}
+
+bool same_plan(const BlockPlanOptions &a, const BlockPlanOptions &b) {
+    return a.max_block == b.max_block && a.split_entries == b.split_entries && a.max_split == b.max_split &&
+           a.split_gather == b.split_gather && a.wide_level_tasks == b.wide_level_tasks &&
+           a.wide_min_split_entries == b.wide_min_split_entries && a.wide_fwd_rows_big == b.wide_fwd_rows_big;
+}
+
+// Rebuilds and uploads the block plan for base options `base` (environment
+// overrides on top) when they differ from the current ones.
+int replan(cholespy_gpu_solver *s, const double *val, const BlockPlanOptions &base) {
+    if (same_plan(base, s->plan_base) && s->uploaded) return CHOLESPY_OK;
+    HIP_TRY(hipDeviceSynchronize());
+    BlockPlan P;
+    try {
+        P = build_block_plan(s->n, s->host_colptr.data(), s->host_rowidx.data(), val, plan_options(base));
+    } catch (const std::exception &ex) {
+        return fail(CHOLESPY_ERR_INVALID_ARGUMENT, ex.what());
+    }
+    int rc = upload_plan(s, P);
+    if (rc != CHOLESPY_OK) return rc;
+    std::vector<double>().swap(P.mfwd);
+    std::vector<double>().swap(P.mbwd);
+    s->host_plan = std::move(P);
+    s->plan_base = base;
+    s->buffers_path = -1;
+    return CHOLESPY_OK;
+}
+
+int apply_config(cholespy_gpu_solver *s, const double *val, int k, const TuneConfig &c) {
+    int rc = replan(s, val, c.opt);
+    if (rc != CHOLESPY_OK) return rc;
+    if (c.path == CHOLESPY_PATH_ROWS && (!s->analysed[0] || !s->analysed[1])) {
+        rc = cholespy_solver_analyze(s, 1, nullptr);
+        if (rc == CHOLESPY_OK) rc = cholespy_solver_analyze(s, 0, nullptr);
+        if (rc != CHOLESPY_OK) return rc;
+    }
+    s->path = c.path;
+    s->tile_for_class[tile_slot(block_tile(s->fbytes, k))] = c.tile;
+    if (!std::getenv("CHOLESPY_FUSED_LEAVES")) s->fused_leaves = c.fused;
+    if (!std::getenv("CHOLESPY_BLOCK_QUEUES")) s->queues[4].nq = s->queues[5].nq = c.queues;
+    for (auto &row : s->grid_blocks)
+        for (int &g : row) g = 0;
+    return CHOLESPY_OK;
+}
+
+// Milliseconds per solve of k zero right-hand sides: the fastest of five
+// batches of solves (each batch about a millisecond long or more).
+int time_config(cholespy_gpu_solver *s, int k, void *b, void *x, double &ms) {
+    auto solve = [&]() { return solve_impl(s, s->fbytes, k, b, x, nullptr); };
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int rc = CHOLESPY_OK;
+    float t = 0.f;
+    for (int i = 0; i < 2 && rc == CHOLESPY_OK; ++i) rc = solve();
+    if (rc == CHOLESPY_OK && hipEventRecord(e0, nullptr) == hipSuccess) {
+        rc = solve();
+        if (rc == CHOLESPY_OK && hipEventRecord(e1, nullptr) == hipSuccess && hipEventSynchronize(e1) == hipSuccess)
+            (void)hipEventElapsedTime(&t, e0, e1);
+    }
+    const int batch = std::max(1, std::min(50, (int)std::ceil(1.0 / std::max(1e-3, (double)t))));
+    ms = 1e30;
+    for (int r = 0; r < 5 && rc == CHOLESPY_OK; ++r) {
+        if (hipEventRecord(e0, nullptr) != hipSuccess) break;
+        for (int i = 0; i < batch && rc == CHOLESPY_OK; ++i) rc = solve();
+        if (rc != CHOLESPY_OK || hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            break;
+        float el = 0.f;
+        if (hipEventElapsedTime(&el, e0, e1) == hipSuccess) ms = std::min(ms, (double)el / batch);
+        if (t > 20.f) break;   // a slow candidate (the row schedule of a large system): one batch
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == CHOLESPY_OK) rc = report_give_up(s);
+    return rc;
+}
+
+std::string default_cache_path() {
+    if (const char *p = std::getenv("CHOLESPY_TUNE_CACHE")) return p;
+    std::string dir;
+    if (const char *x = std::getenv("XDG_CACHE_HOME")) dir = x;
+    else if (const char *h = std::getenv("HOME")) dir = std::string(h) + "/.cache";
+    else dir = "/tmp";
+    return dir + "/cholespy_amd_autotune.txt";
+}
+
+bool cache_lookup(const std::string &path, const std::string &key, TuneConfig &c, double &ms) {
+    FILE *f = std::fopen(path.c_str(), "r");
+    if (!f) return false;
+    char line[1024];
+    bool found = false;
+    while (std::fgets(line, sizeof line, f)) {
+        std::string l(line);
+        const size_t sp = l.find(' ');
+        if (sp == std::string::npos || l.compare(0, sp, key) != 0) continue;
+        const size_t msp = l.rfind(" ms=");
+        TuneConfig t;
+        if (msp == std::string::npos || !parse_config(l.substr(sp + 1, msp - sp - 1), t)) continue;
+        c = t;
+        ms = std::atof(l.c_str() + msp + 4);
+        found = true;   // the last matching line wins
+    }
+    std::fclose(f);
+    return found;
+}
+
+void cache_store(const std::string &path, const std::string &key, const TuneConfig &c, double ms) {
+    const size_t slash = path.rfind('/');
+    if (slash != std::string::npos && slash > 0) (void)::mkdir(path.substr(0, slash).c_str(), 0755);
+    FILE *f = std::fopen(path.c_str(), "a");
+    if (!f) return;
+    std::fprintf(f, "%s %s ms=%.6f\n", key.c_str(), format_config(c).c_str(), ms);
+    std::fclose(f);
+}
+
+int longest_chain(const cholespy_gpu_solver *s) {
+    int longest = s->n > 0 ? 1 : 0;
+    for (int j = 0, run = 1; j + 1 < s->n; ++j) {
+        const bool child = s->host_colptr[j + 1] - s->host_colptr[j] > 1 && s->host_rowidx[s->host_colptr[j] + 1] == j + 1;
+        run = child ? run + 1 : 1;
+        longest = std::max(longest, run);
+    }
+    return longest;
+}
+
+}  // namespace
+
+extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *val, int64_t nnz, int n_rhs,
+                                        const char *cache_path, int flags, char *report, size_t report_len) {
+    if (!s || !val) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    if (s->is_batch) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "batches are not tuned");
+    if (!s->uploaded) return fail(CHOLESPY_ERR_STATE, "autotune called before upload_factor");
+    if (nnz != s->nnz) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "nnz does not match the uploaded factor");
+    if (n_rhs < 1 || n_rhs > kMaxRhs) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "n_rhs must be in [1, 128]");
+    int rc = report_give_up(s);
+    if (rc != CHOLESPY_OK) return rc;
+    const int k = n_rhs;
+    const std::string key = tune_key(s, k);
+    const std::string path = cache_path && *cache_path ? cache_path : default_cache_path();
+    TuneConfig best;
+    best.opt = s->plan_base;
+    best.path = s->path;
+    best.fused = s->fused_leaves;
+    best.queues = s->queues[4].nq;
+    double best_ms = 0;
+    auto finish = [&](const char *how, int evaluated, double default_ms) {
+        std::string r = std::string(how) + " " + format_config(best);
+        char buf[160];
+        std::snprintf(buf, sizeof buf, " ms=%.5f default_ms=%.5f candidates=%d key=%s", best_ms, default_ms,
+                      evaluated, key.c_str());
+        r += buf;
+        s->tune_report = r;
+        if (report && report_len) {
+            std::strncpy(report, r.c_str(), report_len - 1);
+            report[report_len - 1] = 0;
+        }
+        return CHOLESPY_OK;
+    };
+    if (!(flags & 1)) {
+        TuneConfig cached;
+        double ms = 0;
+        if (cache_lookup(path, key, cached, ms)) {
+            rc = apply_config(s, val, k, cached);
+            if (rc != CHOLESPY_OK) return rc;
+            best = cached;
+            best_ms = ms;
+            return finish("cached", 0, 0.0);
+        }
+    }
+    // Work vectors for the timing solves (zeros).
+    const size_t bytes = (size_t)s->n * (size_t)k * (size_t)s->fbytes;
+    void *b = nullptr, *x = nullptr;
+    HIP_TRY(hipMalloc(&b, std::max<size_t>(bytes, 1)));
+    if (hipMalloc(&x, std::max<size_t>(bytes, 1)) != hipSuccess) {
+        (void)hipFree(b);
+        return fail(CHOLESPY_ERR_HIP, "autotune: allocation of the timing vectors failed");
+    }
+    (void)hipMemset(b, 0, bytes);
+    (void)hipMemset(x, 0, bytes);
+    int evaluated = 0;
+    auto eval = [&](const TuneConfig &c, double &ms) {
+        int r = apply_config(s, val, k, c);
+        if (r == CHOLESPY_OK) r = time_config(s, k, b, x, ms);
+        if (r == CHOLESPY_ERR_TIMEOUT) {   // a wait gave up: never choose this one
+            ms = 1e30;
+            r = CHOLESPY_OK;
+        }
+        ++evaluated;
+        return r;
+    };
+    rc = eval(best, best_ms);
+    const double default_ms = best_ms;
+    // One knob at a time; a candidate replaces the best when >= 1 % faster.
+    auto try_values = [&](const char *env, const std::vector<int> &values, auto set) {
+        if (rc != CHOLESPY_OK || (env && std::getenv(env))) return;
+        for (int v : values) {
+            TuneConfig c = best;
+            set(c, v);
+            if (format_config(c) == format_config(best)) continue;
+            double ms = 0;
+            rc = eval(c, ms);
+            if (rc != CHOLESPY_OK) return;
+            if (ms < 0.99 * best_ms) {
+                best = c;
+                best_ms = ms;
+            }
+        }
+    };
+    const bool narrow = block_tile(s->fbytes, k) == kNarrowTile;
+    // The schedule: level-scheduled rows against the block passes. The row
+    // schedule walks the elimination tree one row per level and is only a
+    // candidate for small factors: on large ones a row of a dense separator
+    // takes milliseconds and the whole chain seconds.
+    if (s->nnz <= (int64_t)1 << 20)
+        try_values(nullptr, {CHOLESPY_PATH_ROWS}, [](TuneConfig &c, int v) { c.path = v; });
+    if (best.path == CHOLESPY_PATH_BLOCKS) {
+        // Tile: vector-FMA kernels or MFMA panels, and the panel width.
+        std::vector<int> tiles;
+        if (narrow) tiles = {32};
+        else if (k <= 32) tiles = {64};
+        else if (k <= 64) tiles = {128};
+        try_values(nullptr, tiles, [](TuneConfig &c, int v) { c.tile = v; });
+        const bool wide = !(narrow && best.tile == 0);
+        // Chain cap: powers of two from 1024 up to the longest chain (4096 at most).
+        std::vector<int> caps;
+        const int chain = longest_chain(s);
+        for (int cap = 1024; cap <= 4096 && cap <= std::max(1024, chain); cap *= 2) caps.push_back(cap);
+        try_values("CHOLESPY_MAX_BLOCK", caps, [](TuneConfig &c, int v) { c.opt.max_block = v; });
+        try_values("CHOLESPY_SPLIT_ENTRIES", {2048, 8192}, [](TuneConfig &c, int v) { c.opt.split_entries = v; });
+        try_values("CHOLESPY_MAX_SPLIT", {4, 16}, [](TuneConfig &c, int v) { c.opt.max_split = v; });
+        if (!wide) {
+            try_values("CHOLESPY_SPLIT_GATHER", {33, 129}, [](TuneConfig &c, int v) { c.opt.split_gather = v; });
+            try_values("CHOLESPY_FUSED_LEAVES", {0, 1, 2}, [](TuneConfig &c, int v) { c.fused = v; });
+        } else {
+            try_values("CHOLESPY_WIDE_LEVEL_TASKS", {256, 1024},
+                       [](TuneConfig &c, int v) { c.opt.wide_level_tasks = v; });
+            try_values("CHOLESPY_WIDE_MIN_SPLIT_ENTRIES", {256, 1024},
+                       [](TuneConfig &c, int v) { c.opt.wide_min_split_entries = v; });
+            try_values("CHOLESPY_WIDE_FWD_ROWS_BIG", {32}, [](TuneConfig &c, int v) { c.opt.wide_fwd_rows_big = v; });
+        }
+        try_values("CHOLESPY_BLOCK_QUEUES", {8}, [](TuneConfig &c, int v) { c.queues = v; });
+    }
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b);
+    (void)hipFree(x);
+    if (rc != CHOLESPY_OK) return rc;
+    rc = apply_config(s, val, k, best);
+    if (rc != CHOLESPY_OK) return rc;
+    if (!(flags & 2)) cache_store(path, key, best, best_ms);
+    return finish("tuned", evaluated, default_ms);
+}
+
+extern "C" int cholespy_solver_tune_report(const cholespy_gpu_solver *s, char *report, size_t report_len) {
+    if (!s || !report || !report_len) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null argument");
+    std::strncpy(report, s->tune_report.c_str(), report_len - 1);
+    report[report_len - 1] = 0;
+    return CHOLESPY_OK;
+}

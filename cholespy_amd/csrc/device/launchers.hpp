@@ -127,6 +127,7 @@ struct BlockPassBuffers {
     // leaf_queues with up to 64 queues) instead of a launch of their own.
     bool fused_leaves = false;
     QueueState *leaf_queues = nullptr;
+    int tile = 0;                  // tile width of the launch (0: block_tile's rule for max_k)
 };
 constexpr size_t kLeafQueueWords = 64 * 16 + 16;   // leaf-phase queue counters (padded)
 // Dynamic LDS of a block pass: one chunk of the vector it stages (r_B of

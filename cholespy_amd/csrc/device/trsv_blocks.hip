@@ -649,7 +649,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     BlockPassArgs<F> a;
     a.ntasks = bf.ntasks;
     a.k = bf.k;
-    const int tile = block_tile(sizeof(F), bf.max_k);
+    const int tile = bf.tile ? bf.tile : block_tile(sizeof(F), bf.max_k);
     a.chunk = block_chunk(sizeof(F), kNarrowTile);
     a.lds_bytes = 0;
     a.task = bf.task;

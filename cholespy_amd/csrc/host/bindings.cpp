@@ -361,6 +361,44 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
         .def("_n_levels", [](const Class &self, bool lower) { return self.n_levels(lower); },
              py::arg("lower"))
         .def("_check", [](const Class &self) { self.check(); })
+        .def(
+            "autotune",
+            [](Class &self, int n_rhs, py::object cache, bool force, bool write) {
+                const std::string path = cache.is_none() ? std::string() : py::str(cache).cast<std::string>();
+                std::string report;
+                {
+                    py::gil_scoped_release release;
+                    report = self.autotune(n_rhs, path, (force ? 1 : 0) | (write ? 0 : 2));
+                }
+                // "tuned|cached key=value ..." -> dict
+                py::dict d;
+                size_t pos = 0;
+                bool first = true;
+                while (pos < report.size()) {
+                    size_t end = report.find(' ', pos);
+                    if (end == std::string::npos) end = report.size();
+                    const std::string tok = report.substr(pos, end - pos);
+                    pos = end + 1;
+                    if (first) {
+                        d["source"] = tok;
+                        first = false;
+                        continue;
+                    }
+                    const size_t eq = tok.find('=');
+                    if (eq == std::string::npos) continue;
+                    const std::string key = tok.substr(0, eq), val = tok.substr(eq + 1);
+                    if (key == "key") d[key.c_str()] = val;
+                    else if (key == "ms" || (key.size() > 3 && key.compare(key.size() - 3, 3, "_ms") == 0))
+                        d[key.c_str()] = std::stod(val);
+                    else d[key.c_str()] = std::stoi(val);
+                }
+                return d;
+            },
+            py::arg("n_rhs"), py::arg("cache") = py::none(), py::arg("force") = false, py::arg("write") = true,
+            "Times candidate solve configurations for n_rhs right-hand sides on the GPU and keeps the\n"
+            "fastest (schedule, tile, block-plan shape, leaf phase, ticket queues); the choice is\n"
+            "cached on disk per factor pattern, precision, right-hand-side class and device.\n"
+            "Returns the chosen configuration and its time per solve. GPU solvers only.")
         // Watchdog test hooks (cholespy_solver_set_spin_limit / _raise_abort).
         .def("_set_spin_limit", [](Class &self, uint32_t limit) { throw_on_error(cholespy_solver_set_spin_limit(self.gpu(), limit)); },
              py::arg("limit"))

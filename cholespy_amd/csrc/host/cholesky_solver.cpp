@@ -136,6 +136,17 @@ void CholeskySolver<Float>::solve_refined(int n_rhs, const double *b, double *x,
 }
 
 template <typename Float>
+std::string CholeskySolver<Float>::autotune(int n_rhs, const std::string &cache_path, int flags) {
+    if (!gpu_) throw std::invalid_argument("Autotuning only exists for GPU solvers.");
+    if (n_rhs > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
+    char report[512] = {0};
+    throw_on_error(cholespy_solver_autotune(gpu_, factor_.val.data(), factor_.nnz(), n_rhs,
+                                            cache_path.empty() ? nullptr : cache_path.c_str(), flags, report,
+                                            sizeof report));
+    return report;
+}
+
+template <typename Float>
 void CholeskySolver<Float>::set_path(int path) {
     if (!gpu_) throw std::invalid_argument("Solve paths only exist for GPU solvers.");
     if (path == CHOLESPY_PATH_ROWS) analyse();

@@ -6,6 +6,7 @@
 #pragma once
 #include <cstdint>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "../../../include/cholespy_amd.h"
@@ -62,6 +63,9 @@ public:
     // FP32 solve + FP64 refinement on the GPU (float32 solvers): b, x are
     // device addresses of row-major double arrays.
     void solve_refined(int n_rhs, const double *b, double *x, int iterations);
+    // Device autotuning for n_rhs right-hand sides (cholespy_solver_autotune);
+    // returns the summary line.
+    std::string autotune(int n_rhs, const std::string &cache_path, int flags);
     // Selects the GPU schedule (CHOLESPY_PATH_ROWS / CHOLESPY_PATH_BLOCKS).
     void set_path(int path);
     std::vector<int64_t> plan_info() const;

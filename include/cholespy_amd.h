@@ -235,6 +235,28 @@ int cholespy_batch_solve_f64(cholespy_gpu_solver *batch, const int *n_rhs, const
  * No counterpart in the reference, which has the row schedule only. */
 int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);
 
+/* Autotuner (the reference has no counterpart: it launches one fixed
+ * shape, src/cholesky_solver.cpp:409-412). Times candidate configurations
+ * of this handle for n_rhs right-hand sides on the device and keeps the
+ * fastest: the schedule (CHOLESPY_PATH_ROWS or _BLOCKS), the tile
+ * (vector-FMA or MFMA panel kernels and the panel width), the block plan's
+ * shape (chain cap, split sizes, gather threshold, the wide list's level
+ * targets; the plan is rebuilt from val, the values of L as given to
+ * cholespy_solver_upload_factor), the leaf phase and the ticket queues.
+ * Coordinate descent from the defaults, keeping a change only when it is
+ * at least 1 % faster. Knobs set through CHOLESPY_* environment variables
+ * are not varied. The winner is appended to a text cache (cache_path, or
+ * $CHOLESPY_TUNE_CACHE, or ~/.cache/cholespy_amd_autotune.txt) under a key
+ * of the factor's pattern, the precision, the right-hand-side class and the
+ * device, and a later call with the same key applies the cached choice
+ * without timing. flags: bit 0 = ignore the cache, bit 1 = do not write it.
+ * report (if not NULL) receives a one-line summary. Synchronous; allocates
+ * two n x n_rhs timing vectors. Single-system handles only. */
+int cholespy_solver_autotune(cholespy_gpu_solver *solver, const double *val, int64_t nnz, int n_rhs,
+                             const char *cache_path, int flags, char *report, size_t report_len);
+/* The summary line of the handle's last autotune ("" if none). */
+int cholespy_solver_tune_report(const cholespy_gpu_solver *solver, char *report, size_t report_len);
+
 /* Block-plan statistics: info[0..count) = {blocks, dense entries of all
  * M_B, forward block levels, backward block levels, forward tasks, backward
  * tasks, partial slots, largest m + |R|, forward tasks and backward tasks of
