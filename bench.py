@@ -21,15 +21,16 @@ buffer is overwritten before each solve so the factor is not served from
 the 256 MB Infinity Cache.
 
 roofline: the dominant pass of the headline path (the triangular pass that
-takes longer). A pass at k <= 4 is two launches on one stream: the leaf
-kernel (blocks without child blocks, one wavefront per task) and the block
-kernel over the other blocks; forward runs the leaf launch first, backward
-last. The pass is timed with HIP events recorded between the two passes of
-complete solves on the stream they run on (the null stream); achieved =
-algorithmic bytes of that pass / its average time. traffic = the HBM bytes
-rocprofv3 counted for the pass's kernels on this workload (FETCH_SIZE and
-WRITE_SIZE per dispatch, scaled by the counts of a copy of known size, summed
-over the pass's two kernels), read from the summary kept beside this file in
+takes longer). A pass at k <= 4 is one launch of the block kernel, whose
+leaf phase runs the leaf blocks' tasks (blocks without child blocks, one
+wavefront per task) before (forward) or after (backward) the other blocks'
+tasks; with the leaf phase off it is two launches. The pass is timed with
+HIP events recorded between the two passes of complete solves on the stream
+they run on (the null stream); achieved = algorithmic bytes of that pass /
+its average time. traffic = the HBM bytes rocprofv3 counted for the pass's
+kernels on this workload (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by
+the counts of a copy of known size, summed over the pass's kernels), read
+from the summary kept beside this file in
 profiles/ when it names the same kernels.
 cpu_baseline: the oracle's FP64 simplicial solve (oracle/oracle.c,
 oracle_host_solve: what cholmod_solve does for a simplicial LL' factor) on
@@ -147,6 +148,15 @@ def main():
     sf = CholeskySolverF(n, ii, jj, xx, MatrixType.COO)
     sd = CholeskySolverD(n, ii, jj, xx, MatrixType.COO)
     log(f"setup (2 factorisations + analyses + plans): {time.time() - t0:.2f}s")
+    # Device autotuning for this system and k (fresh on this box, not
+    # cached). The --profile run keeps the defaults, so the profiler sees
+    # only the configuration it measures.
+    tuned = {}
+    if not args.profile:
+        t0 = time.time()
+        tuned = {"f64": sd.autotune(k, force=True, write=False), "f32": sf.autotune(k, force=True, write=False)}
+        log(f"autotune ({time.time() - t0:.1f}s): " + "; ".join(
+            f"{p}: {r['ms']:.4f} ms (defaults {r['default_ms']:.4f})" for p, r in tuned.items()))
     fac = sd._factor()
     nnz_l = len(fac["val"])
     nnz_a = len(sd._matrix()["val"]) * 2 - n
@@ -287,8 +297,14 @@ def main():
     torch.cuda.synchronize()
     solver._check()
     tname = "<float,4>" if s_bytes == 4 else "<double,4>"
-    pass_kernels = ["leaf_forward" + tname, "block_forward" + tname] if dom == 0 else \
-                   ["block_backward" + tname, "leaf_backward" + tname]
+    # A pass is one launch when its leaf tasks run in the block launch's leaf
+    # phase (the default), else a leaf launch and a block launch.
+    fused = tuned.get("f32" if s_bytes == 4 else "f64", {}).get("fused", 3)
+    if (fused >> (0 if dom == 0 else 1)) & 1:
+        pass_kernels = ["block_forward" + tname] if dom == 0 else ["block_backward" + tname]
+    else:
+        pass_kernels = ["leaf_forward" + tname, "block_forward" + tname] if dom == 0 else \
+                       ["block_backward" + tname, "leaf_backward" + tname]
     kernel = ("forward" if dom == 0 else "backward") + " pass: " + " + ".join(pass_kernels)
     workload_name = f"mesh smoothing I+10(D-W) on icosphere({args.level})"
     traffic = measured_traffic(pass_kernels, workload_name, k)
@@ -305,6 +321,8 @@ def main():
                              torch.tensor(v6, device=dev), MatrixType.COO)
         b6 = torch.rand(n6, 128, device=dev)
         x6 = torch.zeros_like(b6)
+        ms6_default = time_events(lambda: s6.solve(b6, x6), 20)
+        tune6 = s6.autotune(128, force=True, write=False)
         ms6 = time_events(lambda: s6.solve(b6, x6), 20)
         ms6z = time_events(lambda: (torch.zeros_like(b6), s6.solve(b6, x6)), 20)
         s6._check()
@@ -329,6 +347,8 @@ def main():
             torch.cuda.synchronize()
             refac.append((time.perf_counter() - t0) * 1e3)
         ref_bench = {"workload": "icosphere(6), n=40962, lambda=20, k=128, float32", "ms_solve": ms6,
+                     "ms_solve_defaults": ms6_default,
+                     "autotune": {kk: v for kk, v in tune6.items() if kk != "key"},
                      "ms_solve_with_zeros_like": ms6z, "reference_ms_titan_rtx": 6.42,
                      "speedup_vs_reference": 6.42 / ms6z,
                      "ms_construct": float(np.median(construct)), "ms_refactorize": float(np.median(refac)),
@@ -400,6 +420,7 @@ def main():
             "refined_vs_f64": results["f64"]["ms"] / results["refined"]["ms"],
             "solve_roofline_frac": solve_bytes / (ms_step * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "block_plan": plan,
+            "autotune": {p: {kk: v for kk, v in r.items() if kk != "key"} for p, r in tuned.items()},
             "reference_benchmark": ref_bench,
             "parallelism": "single GPU",
         },

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kRowBlock) void level_analysis(int n, const int *__
             const int lo = LOWER ? rs : rs + 1;
             const int hi = LOWER ? re - 1 : re;
             int lv = 0;
-            for (int i = lo + lane; i < hi; i += kWave) lv = max(lv, wait_nonzero(&lvl1[cols[i]], status, qa.spin_limit));
+            for (int i = lo + lane; i < hi; i += kWave) lv = max(lv, wait_nonzero(&lvl1[cols[i]], status, qa));
             lv = wave_max(lv);
             if (lane == 0) store_int(&lvl1[row], lv + 1);
         }
@@ -73,6 +73,7 @@ void QueueState::fill(QueueArgs &qa) const {
     qa.ctr = ctr;
     qa.nq = nq;
     qa.spin_limit = spin_limit;
+    qa.abort = abort;
 }
 
 int max_coresident_blocks(const void *kernel, int block, size_t lds_bytes) {

@@ -107,8 +107,8 @@ __device__ __forceinline__ Task load_task(const BlockPassArgs<F> &a, int tk) {
 // Value of a published entry: one agent-scope load, and the bounded wait of
 // common.hpp only when it still holds the sentinel.
 template <typename F>
-__device__ __forceinline__ F ready_value(const F *p, typename Bits<F>::U u, int *status, uint32_t limit) {
-    return u == Bits<F>::sentinel ? wait_value(p, status, limit) : from_bits<F>(u);
+__device__ __forceinline__ F ready_value(const F *p, typename Bits<F>::U u, int *status, const QueueArgs &qa) {
+    return u == Bits<F>::sentinel ? wait_value(p, status, qa) : from_bits<F>(u);
 }
 
 // Split epilogue: the split that adds its group's results is split
@@ -145,7 +145,7 @@ __device__ __forceinline__ void combine_splits(const BlockPassArgs<F> &a, F *sp,
         F v = F(0);
 #pragma unroll
         for (int s = 0; s < kMaxSplit; ++s)
-            if (s < nsplit) v += ready_value(p0 + s * stride, u[s], a.status, a.qa.spin_limit);
+            if (s < nsplit) v += ready_value(p0 + s * stride, u[s], a.status, a.qa);
         const F sent = from_bits<F>(Bits<F>::sentinel);
 #pragma unroll
         for (int s = 0; s < kMaxSplit; ++s)
@@ -176,7 +176,7 @@ __device__ __forceinline__ void combine_splits_own(const BlockPassArgs<F> &a, F 
         F v = F(0);
 #pragma unroll
         for (int s = 0; s < kMaxSplit - 1; ++s)
-            if (s < nsplit - 1) v += ready_value(p0 + s * stride, u[s], a.status, a.qa.spin_limit);
+            if (s < nsplit - 1) v += ready_value(p0 + s * stride, u[s], a.status, a.qa);
         v += own[i * KT + t];
         const F sent = from_bits<F>(Bits<F>::sentinel);
 #pragma unroll
@@ -289,7 +289,7 @@ __device__ __forceinline__ void gather_partials(const BlockPassArgs<F> &a, const
                     for (int j = 0; j < kValBatch; ++j) {
                         F x = from_bits<F>(u[q][j]);
                         if (sl[q][h + j] >= 0 && u[q][j] == Bits<F>::sentinel)
-                            x = wait_value(&pv[(long long)sl[q][h + j] * k + col[q]], a.status, a.qa.spin_limit);
+                            x = wait_value(&pv[(long long)sl[q][h + j] * k + col[q]], a.status, a.qa);
                         v[q] += x;
                     }
             }
@@ -339,7 +339,7 @@ __device__ __forceinline__ void gather_rows(const BlockPassArgs<F> &a, F *dst, i
             const int idx = base + j * kTaskThreads;
             if (idx >= total) break;
             F v = from_bits<F>(u[j]);
-            if (pub[j] && u[j] == Bits<F>::sentinel) v = wait_value(ad[j], a.status, a.qa.spin_limit);
+            if (pub[j] && u[j] == Bits<F>::sentinel) v = wait_value(ad[j], a.status, a.qa);
             const int i = idx / width;
             dst[i * stride + (idx - i * width)] = v;
         }

@@ -38,7 +38,8 @@ namespace {
 
 // Ticket counters of a handle: six queue sets of kQueueWords (analysis,
 // row passes, block passes) and two leaf-phase sets of kLeafQueueWords.
-constexpr size_t kAllQueueWords = 6 * kQueueWords + 2 * kLeafQueueWords;
+// The device copy of the abort word follows them (its own 64-byte line).
+constexpr size_t kAllQueueWords = 6 * kQueueWords + 2 * kLeafQueueWords + 16;
 
 thread_local std::string g_error;
 bool g_initialised = false;
@@ -151,6 +152,7 @@ struct cholespy_gpu_solver {
     // device address the kernels get, hstatus the host one.
     int *status = nullptr;
     volatile int *hstatus = nullptr;
+    int *dabort = nullptr;                // device copy of the abort word (in qctr)
     // A forward pass (cholespy_solver_solve_pass 0) whose backward pass has
     // not run yet: its partial sums, gathered right-hand sides and y are
     // still published, so the next forward pass must start from a refill.
@@ -174,6 +176,8 @@ struct cholespy_gpu_solver {
     float *r32 = nullptr;                 // FP32 right-hand side of correction solves
     size_t r32_elems = 0;
     double *norms = nullptr;              // refinement stopping test: [0, 128) residual, [128, 256) b
+    double *hnorms = nullptr;             // their host-mapped copy (pinned)
+    double *hnorms_dev = nullptr;         // its device address
 
     int grid_rows[2][2] = {{0, 0}, {0, 0}};   // [lower][k > 64]
     int grid_blocks[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};   // [forward][tile_slot]
@@ -482,6 +486,8 @@ int cholespy_solver_create(int n, int float_bytes, cholespy_gpu_solver **out) {
             s->queues[6 + i].ctr = s->qctr + 6 * kQueueWords + (size_t)i * kLeafQueueWords;
             s->queues[6 + i].nq = kMaxLeafQueues;
         }
+        s->dabort = reinterpret_cast<int *>(s->qctr + kAllQueueWords - 16);
+        for (QueueState &q : s->queues) q.abort = s->dabort;
     }
     // Block-pass ticket queues (tuning sweeps: CHOLESPY_BLOCK_QUEUES, 1..16).
     if (const char *v = std::getenv("CHOLESPY_BLOCK_QUEUES"))
@@ -532,6 +538,8 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->avals);
     dev_free(s->r32);
     dev_free(s->norms);
+    if (s->hnorms) (void)hipHostFree(s->hnorms);
+    s->hnorms = nullptr;
     dev_free(s->blk_sys);
     dev_free(s->sys_k);
     dev_free(s->sys_vbase);
@@ -690,6 +698,7 @@ int cholespy_solver_analyze(cholespy_gpu_solver *s, int lower, int *n_levels) {
     // cleared (one could be giving up right now).
     HIP_TRY(hipDeviceSynchronize());
     s->hstatus[0] = s->hstatus[1] = 0;
+    HIP_TRY(hipMemset(s->dabort, 0, sizeof(int)));
     const int cap = max_coresident_blocks(level_analysis_kernel(lower != 0), kRowBlock, 0);
     if (cap <= 0) return fail(CHOLESPY_ERR_HIP, "occupancy query failed for level analysis");
     const int grid = std::max(1, std::min(cap, (n + 4 * kRowChunk - 1) / (4 * kRowChunk)));
@@ -699,6 +708,7 @@ int cholespy_solver_analyze(cholespy_gpu_solver *s, int lower, int *n_levels) {
     HIP_TRY(hipMemcpy(lvl1.data(), s->lvl1, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
     if (s->hstatus[0] != 0 || s->hstatus[1] != 0) {
         s->hstatus[0] = s->hstatus[1] = 0;
+        HIP_TRY(hipMemset(s->qctr, 0, kAllQueueWords * sizeof(unsigned int)));
         return fail(CHOLESPY_ERR_TIMEOUT, "level analysis exceeded its bounded wait");
     }
     // Stable counting sort of rows by level (src/cholesky_solver.cpp:348-367).
@@ -750,6 +760,8 @@ int cholespy_solver_set_spin_limit(cholespy_gpu_solver *s, uint32_t limit) {
 int cholespy_solver_raise_abort(cholespy_gpu_solver *s) {
     if (!s) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "null solver");
     s->hstatus[1] = 1;
+    const int one = 1;
+    HIP_TRY(hipMemcpy(s->dabort, &one, sizeof one, hipMemcpyHostToDevice));
     return CHOLESPY_OK;
 }
 
@@ -1056,15 +1068,23 @@ int refined_impl(cholespy_gpu_solver *s, int k, const double *b, double *x, int 
         return CHOLESPY_OK;
     }
     if (!s->norms) HIP_TRY(dev_alloc(&s->norms, 2 * (size_t)kMaxRhs));
+    if (!s->hnorms) {
+        void *hp = nullptr, *dp = nullptr;
+        HIP_TRY(hipHostMalloc(&hp, 2 * (size_t)kMaxRhs * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent));
+        s->hnorms = static_cast<double *>(hp);
+        HIP_TRY(hipHostGetDevicePointer(&dp, hp, 0));
+        s->hnorms_dev = static_cast<double *>(dp);
+    }
     double *rn = s->norms, *bn = s->norms + kMaxRhs;
     HIP_TRY(launch_column_norms2((size_t)s->n, k, b, bn, st));
-    std::vector<double> h(2 * (size_t)kMaxRhs, 0.0);
+    const volatile double *h = s->hnorms;
     int it = 0;
     double rel = 0.0;
     for (;; ++it) {
         HIP_TRY(launch_zero_f64(rn, k, st));
         HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, rn, st));
-        HIP_TRY(hipMemcpyAsync(h.data(), s->norms, 2 * (size_t)kMaxRhs * sizeof(double), hipMemcpyDeviceToHost, st));
+        // The norms straight into pinned host memory (no copy engine), then wait.
+        HIP_TRY(launch_copy_out_f64(s->hnorms_dev, s->norms, 2 * kMaxRhs, st));
         HIP_TRY(hipStreamSynchronize(st));
         rc = report_give_up(s);
         if (rc != CHOLESPY_OK) return rc;

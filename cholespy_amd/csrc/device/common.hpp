@@ -19,8 +19,10 @@
 // that gives up returns a quiet NaN, so the results it feeds are NaN, never
 // finite and wrong. The two status words live in host memory mapped into
 // the device (fine-grained, coherent): kernels only store to them, with
-// system-scope stores on the give-up path, and read the abort word every
-// 256 polls of a long wait; the host reads them with no device call.
+// system-scope stores on the give-up path, and the host reads them with no
+// device call. Long waits read a device-memory copy of the abort word every
+// 256 polls (a host-memory read from thousands of waiting wavefronts would
+// load the PCIe link the whole time).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -86,15 +88,21 @@ __device__ __forceinline__ void store_int(int *p, int v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Dynamic work queues (below) and the watchdog's launch arguments.
+struct QueueArgs {
+    unsigned int *ctr;    // nq counters kQueueStride apart, then the done counter
+    int nq;
+    uint32_t spin_limit;  // poll budget of every bounded wait of the launch
+    int *abort;           // device copy of the abort word (read by long waits)
+};
+
 // status[0] = error code, status[1] = abort flag, in mapped host memory.
-// Plain system-scope stores and loads only (no read-modify-write on host
-// memory): every writer stores the same code.
-__device__ __forceinline__ int load_status(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-}
-__device__ __forceinline__ void raise_timeout(int *status) {
+// Plain system-scope stores only (no read-modify-write on host memory):
+// every writer stores the same code. The device abort word is raised too.
+__device__ __forceinline__ void raise_timeout(int *status, const QueueArgs &qa) {
     __hip_atomic_store(status, kStatusTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(status + 1, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(qa.abort, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Default poll budget of one wait (about 10 s of polling); a handle may set a
@@ -113,7 +121,7 @@ __device__ __forceinline__ F give_up_value() {
 // `limit` polls it records the timeout and returns a quiet NaN; it also
 // returns a quiet NaN when it sees the abort word (checked every 256 polls).
 template <typename F>
-__device__ __forceinline__ F wait_value(const F *p, int *status, uint32_t limit) {
+__device__ __forceinline__ F wait_value(const F *p, int *status, const QueueArgs &qa) {
     using U = typename Bits<F>::U;
     U u = load_bits(p);
     if (u != Bits<F>::sentinel) return from_bits<F>(u);
@@ -122,17 +130,17 @@ __device__ __forceinline__ F wait_value(const F *p, int *status, uint32_t limit)
         __builtin_amdgcn_s_sleep(CHOLESPY_SPIN_SLEEP);
         u = load_bits(p);
         if (u != Bits<F>::sentinel) return from_bits<F>(u);
-        if (++spins >= limit) {
-            raise_timeout(status);
+        if (++spins >= qa.spin_limit) {
+            raise_timeout(status, qa);
             return give_up_value<F>();
         }
-        if ((spins & 255u) == 0 && load_status(status + 1)) return give_up_value<F>();
+        if ((spins & 255u) == 0 && load_int(qa.abort)) return give_up_value<F>();
     }
 }
 
 // Waits until *p is non-zero and returns it (same watchdog as wait_value;
 // returns 1 when it gives up, and the status word says so).
-__device__ __forceinline__ int wait_nonzero(const int *p, int *status, uint32_t limit) {
+__device__ __forceinline__ int wait_nonzero(const int *p, int *status, const QueueArgs &qa) {
     int v = load_int(p);
     if (v != 0) return v;
     uint32_t spins = 0;
@@ -140,11 +148,11 @@ __device__ __forceinline__ int wait_nonzero(const int *p, int *status, uint32_t 
         __builtin_amdgcn_s_sleep(1);
         v = load_int(p);
         if (v != 0) return v;
-        if (++spins >= limit) {
-            raise_timeout(status);
+        if (++spins >= qa.spin_limit) {
+            raise_timeout(status, qa);
             return 1;
         }
-        if ((spins & 255u) == 0 && load_status(status + 1)) return 1;
+        if ((spins & 255u) == 0 && load_int(qa.abort)) return 1;
     }
 }
 
@@ -162,12 +170,6 @@ __device__ __forceinline__ int wait_nonzero(const int *p, int *status, uint32_t 
 constexpr int kMaxQueues = 16;
 constexpr int kMaxLeafQueues = 64;   // queues of a fused launch's leaf phase
 constexpr int kQueueStride = 16;   // 128 bytes between counters
-
-struct QueueArgs {
-    unsigned int *ctr;    // nq counters kQueueStride apart, then the done counter
-    int nq;
-    uint32_t spin_limit;  // poll budget of every bounded wait of the launch
-};
 
 // Next ticket of queue q, or -1 when the queue is exhausted (n tickets in all).
 __device__ __forceinline__ int take_ticket(const QueueArgs &qa, int q, int n) {

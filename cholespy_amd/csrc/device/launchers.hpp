@@ -17,6 +17,7 @@ struct QueueState {
     unsigned int *ctr = nullptr;
     int nq = 16;
     uint32_t spin_limit = 1u << 23;   // poll budget of each bounded wait (common.hpp kSpinLimit)
+    int *abort = nullptr;             // device copy of the handle's abort word
     void fill(QueueArgs &qa) const;
 };
 constexpr size_t kQueueWords = 16 * 16 + 16;   // counters + done word (padded)
@@ -147,6 +148,8 @@ hipError_t launch_residual_f64(int n, int k, const int *rowptr, const int *cols,
 // norm2[t] = sum_i v[i,t]^2 for a row-major (rows x k) FP64 array.
 hipError_t launch_column_norms2(size_t rows, int k, const double *v, double *norm2, hipStream_t stream);
 hipError_t launch_zero_f64(double *p, int count, hipStream_t stream);
+// host_mapped[i] = src[i], i < count, into host memory mapped into the device.
+hipError_t launch_copy_out_f64(double *host_mapped, const double *src, int count, hipStream_t stream);
 
 // dL/d(stored entries of A) from the adjoint lam and the solution x (grad.hip).
 // Entries with an index outside [0, n) get NaN and are not read through.

@@ -19,13 +19,19 @@ __global__ void cast_to_f32(const double *__restrict__ in, float *__restrict__ o
         out[i] = (float)in[i];
 }
 
-// Per-column sums of squares: each thread adds its squares into the
-// workgroup's LDS sums (one per column), then one global FP64 atomic per
-// column and workgroup.
-__device__ __forceinline__ void add_column_squares(double *lsum, int k, double *norm2) {
+// Per-column sums of squares. The grid-stride loops below step by a
+// multiple of k, so every thread stays on one column t and keeps its sum in
+// a register; at the end each thread adds it to the workgroup's LDS sum of
+// its column, and one global FP64 atomic per column and workgroup follows.
+__device__ __forceinline__ size_t column_stride(int k) {
+    const size_t threads = (size_t)gridDim.x * blockDim.x;
+    return threads >= (size_t)k ? threads - threads % (size_t)k : (size_t)k;
+}
+__device__ __forceinline__ void add_column_squares(double *lsum, int k, int t, double mine, double *norm2) {
+    if (t >= 0 && mine != 0.0) atomicAdd(&lsum[t], mine);
     __syncthreads();
-    for (int t = threadIdx.x; t < k; t += blockDim.x)
-        if (lsum[t] != 0.0) atomicAdd(&norm2[t], lsum[t]);
+    for (int c = threadIdx.x; c < k; c += blockDim.x)
+        if (lsum[c] != 0.0) atomicAdd(&norm2[c], lsum[c]);
 }
 
 // One thread per (row, right-hand side): consecutive threads share a row,
@@ -39,16 +45,17 @@ __global__ void residual_f64(int n, int k, const int *__restrict__ rowptr, const
         for (int t = threadIdx.x; t < k; t += blockDim.x) lsum[t] = 0.0;
         __syncthreads();
     }
-    const size_t total = (size_t)n * k;
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
+    const size_t total = (size_t)n * k, stride = column_stride(k);
+    const size_t idx0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    double mine = 0.0;
+    for (size_t idx = idx0; idx < total && idx0 < stride; idx += stride) {
         const int row = (int)(idx / k), t = (int)(idx % k);
         double s = b[idx];
         for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) s = fma(-vals[p], x[(size_t)cols[p] * k + t], s);
         r32[idx] = (float)s;
-        if (norm2) atomicAdd(&lsum[t], s * s);
+        mine = fma(s, s, mine);
     }
-    if (norm2) add_column_squares(lsum, k, norm2);
+    if (norm2) add_column_squares(lsum, k, idx0 < stride ? (int)(idx0 % k) : -1, mine, norm2);
 }
 
 // norm2[t] += sum_i v[i,t]^2 for a row-major (n x k) array.
@@ -56,12 +63,16 @@ __global__ void column_squares(size_t total, int k, const double *__restrict__ v
     __shared__ double lsum[kMaxRhs];
     for (int t = threadIdx.x; t < k; t += blockDim.x) lsum[t] = 0.0;
     __syncthreads();
-    for (size_t idx = blockIdx.x * (size_t)blockDim.x + threadIdx.x; idx < total;
-         idx += (size_t)gridDim.x * blockDim.x) {
-        const double e = v[idx];
-        atomicAdd(&lsum[idx % k], e * e);
-    }
-    add_column_squares(lsum, k, norm2);
+    const size_t stride = column_stride(k), idx0 = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    double mine = 0.0;
+    for (size_t idx = idx0; idx < total && idx0 < stride; idx += stride) mine = fma(v[idx], v[idx], mine);
+    add_column_squares(lsum, k, idx0 < stride ? (int)(idx0 % k) : -1, mine, norm2);
+}
+
+// dst[i] = src[i]: device values into host memory mapped into the device.
+__global__ void copy_out_f64(double *dst, const double *src, int count) {
+    for (int i = threadIdx.x; i < count; i += blockDim.x)
+        __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __global__ void zero_f64(double *p, int count) {
@@ -94,6 +105,11 @@ hipError_t launch_column_norms2(size_t rows, int k, const double *v, double *nor
     const size_t count = rows * (size_t)k;
     if (count == 0) return hipGetLastError();
     column_squares<<<grid_for(count, 256), 256, 0, stream>>>(count, k, v, norm2);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_out_f64(double *host_mapped, const double *src, int count, hipStream_t stream) {
+    copy_out_f64<<<1, 256, 0, stream>>>(host_mapped, src, count);
     return hipGetLastError();
 }
 

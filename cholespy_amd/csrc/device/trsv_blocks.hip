@@ -185,7 +185,7 @@ __device__ __forceinline__ void leaf_backward_task(const BlockPassArgs<F> &a, co
             for (int t = 0; t < KT; ++t) u[t] = t < k ? peek_bits(src + t) : U(0);
 #pragma unroll
             for (int t = 0; t < KT; ++t)
-                v[(i - first) * KT + t] = t < k ? ready_value(src + t, u[t], a.status, a.qa.spin_limit) : F(0);
+                v[(i - first) * KT + t] = t < k ? ready_value(src + t, u[t], a.status, a.qa) : F(0);
         }
     }
     wave_sync();
@@ -682,7 +682,7 @@ hipError_t launch_impl(bool forward, const BlockPassBuffers &bf, int grid, Queue
     a.n_leaf = 0;
     a.leaf_trace = nullptr;
     if (bf.leaf_queues) bf.leaf_queues->fill(a.lqa);
-    else a.lqa = QueueArgs{nullptr, 1, a.qa.spin_limit};
+    else a.lqa = QueueArgs{nullptr, 1, a.qa.spin_limit, a.qa.abort};
     if (is_panel_tile(tile)) {
         if constexpr (sizeof(F) == 4) return launch_panel_f32(forward, a, tile / 16, grid, stream);
         else return launch_panel_f64(forward, a, tile / 16, grid, stream);

@@ -211,7 +211,7 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
                     for (int j = 0; j < kSlotGroup; ++j) {
                         const int idx = base + e * kTaskThreads;
                         const int sl = idx < total ? st.slot[ii[e]][g + j] : -1;
-                        if (sl >= 0) v[e] += ready_value(&pv[(long long)sl * k + tt[e]], us[e][j], a.status, a.qa.spin_limit);
+                        if (sl >= 0) v[e] += ready_value(&pv[(long long)sl * k + tt[e]], us[e][j], a.status, a.qa);
                     }
             }
 #pragma unroll
@@ -221,7 +221,7 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
                 const int i = ii[e], t = tt[e];
                 for (int j = kStageSlots; j < st.cnt[i]; ++j) {   // rows with very many contributions
                     const F *p = &pv[(long long)a.cslot[st.first[i] + j] * k + t];
-                    v[e] += ready_value(p, peek_bits(p), a.status, a.qa.spin_limit);
+                    v[e] += ready_value(p, peek_bits(p), a.status, a.qa);
                 }
                 if (to_rbuf)
                     publish(&rb[(long long)(a0 + q0 + i) * k + t], v[e]);
@@ -378,7 +378,7 @@ void block_forward_panel(BlockPassArgs<F> a) {
                             for (int j = 0; j < kMaxTiles; ++j)
                                 if (j < w.ntiles)
                                     acc[j] = Mfma<F>::step(
-                                        av[q], ready_value(rb0 + (long long)col * k + 16 * j * w.tstride, bu[q][j], a.status, a.qa.spin_limit),
+                                        av[q], ready_value(rb0 + (long long)col * k + 16 * j * w.tstride, bu[q][j], a.status, a.qa),
                                         acc[j]);
                         }
                     }
@@ -518,7 +518,7 @@ void block_backward_panel(BlockPassArgs<F> a) {
 #pragma unroll
                             for (int j = 0; j < kMaxTiles; ++j)
                                 if (j < w.ntiles) {
-                                    const F b = pubr[q] && bp[q] ? ready_value(bp[q] + 16 * j * w.tstride, bu[q][j], a.status, a.qa.spin_limit)
+                                    const F b = pubr[q] && bp[q] ? ready_value(bp[q] + 16 * j * w.tstride, bu[q][j], a.status, a.qa)
                                                                  : from_bits<F>(bu[q][j]);
                                     acc[j] = Mfma<F>::step(av[q], b, acc[j]);
                                 }

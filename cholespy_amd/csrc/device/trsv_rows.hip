@@ -109,7 +109,7 @@ __global__ __launch_bounds__(kRowBlock) void row_pass(RowPassArgs<F> a) {
                 for (int q = 0; q < CPL; ++q) {
                     F xv = from_bits<F>(xb[j][q]);
                     if (xb[j][q] == Bits<F>::sentinel)
-                        xv = wait_value(&dep[(size_t)c[j] * k + t + q * kWave], a.status, a.qa.spin_limit);
+                        xv = wait_value(&dep[(size_t)c[j] * k + t + q * kWave], a.status, a.qa);
                     acc[q] = fma(v[j], xv, acc[q]);
                 }
             }
