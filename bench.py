@@ -466,6 +466,9 @@ def bench_precision(args):
                   torch.tensor(values, device=dev))
     sf = CholeskySolverF(n, ii, jj, xx, MatrixType.COO)
     log(f"FP32 solver (FP64 host factor + plan): {time.time() - t0:.1f}s, plan {sf._plan_info()}")
+    t0 = time.time()
+    tune = sf.autotune(k, force=True, write=False)
+    log(f"autotune ({time.time() - t0:.1f}s): {tune['ms']:.4f} ms per FP32 solve (defaults {tune['default_ms']:.4f})")
     fac = sf._factor()
     A = sp.coo_matrix((values, (idx[0], idx[1])), shape=(n, n)).tocsr()
     t0 = time.time()
@@ -534,6 +537,7 @@ def bench_precision(args):
             "ms": {"refined": ms_refined, "f32": ms_f32, "f64": ms_f64},
             "f64": {"residual": residual(x64h), "rel_err": rel_err(x64h)},
             "refined_vs_f64": ms_f64 / ms_refined,
+            "autotune_f32": {kk: v for kk, v in tune.items() if kk != "key"},
         },
     }
     print(json.dumps(out))

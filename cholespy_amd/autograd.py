@@ -50,6 +50,29 @@ def _entry_gradient_host(rows, cols, lam, x):
     return g
 
 
+# (data pointer, version) of index tensors already checked against n: a
+# training loop passes the same rows / cols every step, and the check costs
+# a device synchronisation.
+_checked: dict = {}
+
+
+def _check_indices(rows, cols, n):
+    """IndexError unless every stored entry's row and column lie in [0, n).
+    (The gradient kernel never reads through an out-of-range index either:
+    such an entry's gradient is NaN.)"""
+    if rows.numel() == 0:
+        return
+    key = (rows.data_ptr(), rows._version, cols.data_ptr(), cols._version, rows.numel(), n)
+    if key in _checked:
+        return
+    lo, hi = torch.stack([rows.min(), cols.min(), rows.max(), cols.max()]).view(2, 2).tolist()
+    if min(lo) < 0 or max(hi) >= n:
+        raise IndexError(f"rows / cols hold an index outside [0, {n}): [{min(lo)}, {max(hi)}]")
+    if len(_checked) > 64:
+        _checked.clear()
+    _checked[key] = True
+
+
 class SolveFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, b, values, solver, rows, cols):
@@ -73,12 +96,7 @@ class SolveFunction(torch.autograd.Function):
             if rows is None or cols is None:
                 raise ValueError("gradients for the matrix entries need rows and cols (coo_indices)")
             dtype, device, shape = ctx.values_meta
-            n = x.shape[0]
-            if rows.numel() > 0:
-                lo = min(int(rows.min()), int(cols.min()))
-                hi = max(int(rows.max()), int(cols.max()))
-                if lo < 0 or hi >= n:
-                    raise IndexError(f"rows / cols hold an index outside [0, {n}): [{lo}, {hi}]")
+            _check_indices(rows, cols, x.shape[0])
             if x.is_cuda:
                 gv = torch.empty(shape, dtype=dtype, device=device)
                 _core._entry_gradient(rows, cols, lam, x, gv)

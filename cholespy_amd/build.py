@@ -34,7 +34,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-sign-compare", "-pthread"]
 
 DEVICE_SOURCES = ["analysis.hip", "trsv_rows.hip", "trsv_blocks.hip", "trsv_panel.hip", "refine.hip", "grad.hip", "normalize.hip",
-                  "capi.hip"]
+                  "probe.hip", "capi.hip"]
 PLAN_SOURCES = ["block_plan.cpp"]   # host code linked into libcholespy_amd.so
 HOST_SOURCES = ["sparse_input.cpp", "ordering.cpp", "factor.cpp", "supernodal.cpp", "cholesky_solver.cpp",
                 "bindings.cpp"]

@@ -1081,6 +1081,14 @@ int refined_impl(cholespy_gpu_solver *s, int k, const double *b, double *x, int 
     int it = 0;
     double rel = 0.0;
     for (;; ++it) {
+        if (it == 0 && tol < 1e-6 && iterations > 0) {
+            // An FP32 solve is never that accurate: correct once before the
+            // first check (one synchronisation fewer).
+            HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, nullptr, st));
+            rc = solve_blocks(s, k, s->r32, x, st, 2);
+            if (rc != CHOLESPY_OK) return rc;
+            continue;
+        }
         HIP_TRY(launch_zero_f64(rn, k, st));
         HIP_TRY(launch_residual_f64(s->n, k, s->arowptr, s->acols, s->avals, b, x, s->r32, rn, st));
         // The norms straight into pinned host memory (no copy engine), then wait.
@@ -1595,22 +1603,24 @@ std::string tune_key(const cholespy_gpu_solver *s, int k) {
 
 // Field order of a cache line after the key.
 std::string format_config(const TuneConfig &c) {
-    char buf[256];
+    char buf[320];
     std::snprintf(buf, sizeof buf,
                   "path=%d tile=%d fused=%d queues=%d max_block=%d split_entries=%d max_split=%d split_gather=%d "
-                  "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d",
+                  "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d leaf_max=%d",
                   c.path, c.tile, c.fused, c.queues, c.opt.max_block, c.opt.split_entries, c.opt.max_split,
-                  c.opt.split_gather, c.opt.wide_level_tasks, c.opt.wide_min_split_entries, c.opt.wide_fwd_rows_big);
+                  c.opt.split_gather, c.opt.wide_level_tasks, c.opt.wide_min_split_entries, c.opt.wide_fwd_rows_big,
+                  c.opt.leaf_max);
     return buf;
 }
 
 bool parse_config(const std::string &text, TuneConfig &c) {
-    int v[11];
+    int v[12];
     if (std::sscanf(text.c_str(),
                     "path=%d tile=%d fused=%d queues=%d max_block=%d split_entries=%d max_split=%d split_gather=%d "
-                    "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d",
-                    &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10]) != 11)
+                    "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d leaf_max=%d",
+                    &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11]) != 12)
         return false;
+    if (v[11] < 1 || v[11] > 4096) return false;
     if ((v[0] != CHOLESPY_PATH_ROWS && v[0] != CHOLESPY_PATH_BLOCKS) || v[2] < 0 || v[2] > 3 || v[3] < 1 ||
         v[3] > kMaxQueues || v[4] < 0 || v[5] < 0 || v[6] < 1 || v[6] > 16 || v[7] < 1 || v[8] < 0 ||
         v[9] < 1 || v[10] < 1 || v[10] > 64)
@@ -1626,13 +1636,15 @@ bool parse_config(const std::string &text, TuneConfig &c) {
     c.opt.wide_level_tasks = v[8];
     c.opt.wide_min_split_entries = v[9];
     c.opt.wide_fwd_rows_big = v[10];
+    c.opt.leaf_max = v[11];
     return true;
 }
 
 bool same_plan(const BlockPlanOptions &a, const BlockPlanOptions &b) {
     return a.max_block == b.max_block && a.split_entries == b.split_entries && a.max_split == b.max_split &&
            a.split_gather == b.split_gather && a.wide_level_tasks == b.wide_level_tasks &&
-           a.wide_min_split_entries == b.wide_min_split_entries && a.wide_fwd_rows_big == b.wide_fwd_rows_big;
+           a.wide_min_split_entries == b.wide_min_split_entries && a.wide_fwd_rows_big == b.wide_fwd_rows_big &&
+           a.leaf_max == b.leaf_max;
 }
 
 // Rebuilds and uploads the block plan for base options `base` (environment
@@ -1856,6 +1868,8 @@ extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *va
         const int chain = longest_chain(s);
         for (int cap = 1024; cap <= 4096 && cap <= std::max(1024, chain); cap *= 2) caps.push_back(cap);
         try_values("CHOLESPY_MAX_BLOCK", caps, [](TuneConfig &c, int v) { c.opt.max_block = v; });
+        // Leaf subtrees: larger ones can save a block level.
+        try_values("CHOLESPY_LEAF_MAX", {48, 96}, [](TuneConfig &c, int v) { c.opt.leaf_max = v; });
         try_values("CHOLESPY_SPLIT_ENTRIES", {2048, 8192}, [](TuneConfig &c, int v) { c.opt.split_entries = v; });
         try_values("CHOLESPY_MAX_SPLIT", {4, 16}, [](TuneConfig &c, int v) { c.opt.max_split = v; });
         if (!wide) {
@@ -1878,6 +1892,14 @@ extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *va
     if (rc != CHOLESPY_OK) return rc;
     if (!(flags & 2)) cache_store(path, key, best, best_ms);
     return finish("tuned", evaluated, default_ms);
+}
+
+extern "C" int cholespy_probe_handoff(double *results, int count) {
+    if (!results || count < 6) return fail(CHOLESPY_ERR_INVALID_ARGUMENT, "results needs 6 doubles");
+    int rc = cholespy_init();
+    if (rc != CHOLESPY_OK) return rc;
+    HIP_TRY(probe_handoff(results));
+    return CHOLESPY_OK;
 }
 
 extern "C" int cholespy_solver_tune_report(const cholespy_gpu_solver *s, char *report, size_t report_len) {

@@ -157,6 +157,12 @@ hipError_t launch_entry_gradient(int n, int64_t nnz, const int *rows, const int 
                                  const void *x, int vec_bytes, void *grad, int grad_bytes,
                                  hipStream_t stream);
 
+// Hand-off microbenchmark (probe.hip): res[0..6) = one-way hand-off between
+// two workgroups, relaxed and with release/acquire fences (us; -1 if a wait
+// gave up), dependent agent-scope load (ns), workgroup barrier (ns), ticket
+// atomic on one address under contention (ns per ticket), empty launch (us).
+hipError_t probe_handoff(double *res);
+
 // Fills n words of `bytes` (4 or 8) with the sentinel of that width.
 hipError_t launch_fill_sentinel(void *p, size_t n, int bytes, hipStream_t stream);
 

@@ -598,6 +598,18 @@ void entry_gradient(py::object rows_o, py::object cols_o, py::object lam_o, py::
 }
 
 PYBIND11_MODULE(_cholespy_core, m) {
+    m.def("_probe_handoff", []() {
+        double r[6] = {0};
+        throw_on_error(cholespy_probe_handoff(r, 6));
+        py::dict d;
+        d["handoff_relaxed_us"] = r[0];
+        d["handoff_fenced_us"] = r[1];
+        d["agent_load_ns"] = r[2];
+        d["barrier_ns"] = r[3];
+        d["ticket_atomic_ns"] = r[4];
+        d["launch_us"] = r[5];
+        return d;
+    }, "Microbenchmark of the hand-off costs the block schedule is built around (cholespy_probe_handoff).");
     m.def("_entry_gradient", &entry_gradient, py::arg("rows"), py::arg("cols"), py::arg("lam"),
           py::arg("x"), py::arg("grad"));
     py::enum_<MatrixType>(m, "MatrixType", doc_matrix_type)

@@ -254,6 +254,17 @@ int cholespy_solver_set_path(cholespy_gpu_solver *solver, int path);
  * two n x n_rhs timing vectors. Single-system handles only. */
 int cholespy_solver_autotune(cholespy_gpu_solver *solver, const double *val, int64_t nnz, int n_rhs,
                              const char *cache_path, int flags, char *report, size_t report_len);
+/* Microbenchmark of the costs the block schedule is built around (no
+ * counterpart in the reference; synchronous, allocates its own buffers):
+ * results[0] one-way hand-off of a value between two workgroups through an
+ * agent-scope relaxed store and loads of a sentinel word (us), results[1]
+ * the same with release / acquire fences (us), results[2] one dependent
+ * agent-scope load (ns), results[3] one 256-thread workgroup barrier (ns),
+ * results[4] one ticket fetch-add on a shared address with 1024 workgroups
+ * contending (ns, device-wide), results[5] one back-to-back empty launch
+ * (us). A hand-off whose wait gave up reports -1. count >= 6. */
+int cholespy_probe_handoff(double *results, int count);
+
 /* The summary line of the handle's last autotune ("" if none). */
 int cholespy_solver_tune_report(const cholespy_gpu_solver *solver, char *report, size_t report_len);
 
