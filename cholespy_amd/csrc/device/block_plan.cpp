@@ -37,6 +37,7 @@ struct TaskShape {
     int busy_fwd_rows, busy_split_gather, fine_bwd_cols_big;
     bool leaf_partition;
     int busy_fwd_rows_big;
+    bool critical_order;
 };
 
 struct TaskList {
@@ -48,13 +49,13 @@ TaskShape narrow_shape(const BlockPlanOptions &o) {
     return TaskShape{o.fwd_rows,          o.bwd_cols,          o.split_gather, o.fwd_rows_big,
                      o.bwd_cols_big,      o.split_entries,     o.max_split,    o.level_tasks,
                      o.min_split_entries, o.fwd_rows,          o.split_gather, o.bwd_cols_big,
-                     true,                o.fwd_rows_big};
+                     true,                o.fwd_rows_big,      o.critical_order};
 }
 TaskShape wide_shape(const BlockPlanOptions &o) {
     return TaskShape{o.wide_fwd_rows,          o.bwd_cols,      o.wide_split_gather, o.wide_fwd_rows_big,
                      o.bwd_cols_big,           o.split_entries, o.max_split,         o.wide_level_tasks,
                      o.wide_min_split_entries, o.fwd_rows,      o.split_gather,      o.wide_fine_bwd_cols_big,
-                     false,                    o.fwd_rows_big};
+                     false,                    o.fwd_rows_big,  o.critical_order};
 }
 
 // Tasks of both passes in level order (stable in block index) for one task shape.
@@ -103,8 +104,12 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
     };
     std::vector<int> order(nb);
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return P.level_fwd[x] < P.level_fwd[y]; });
+    // Level order; inside a level the deepest blocks (longest chain to the
+    // root still ahead) first when critical_order is set.
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (P.level_fwd[x] != P.level_fwd[y]) return P.level_fwd[x] < P.level_fwd[y];
+        return opt.critical_order && P.level_bwd[x] > P.level_bwd[y];
+    });
     for (size_t lo = 0; lo < order.size();) {
         // One level at a time: the gather tasks of its large blocks first,
         // then the product tasks, so every task's inputs come from tasks with
@@ -170,8 +175,12 @@ TaskList make_tasks(const BlockPlan &P, const TaskShape &opt) {
     const int fwd_split_rows = next_split;
     next_split = 0;
     std::iota(order.begin(), order.end(), 0);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int x, int y) { return P.level_bwd[x] < P.level_bwd[y]; });
+    // Depth order; inside a depth the blocks with the most levels below them
+    // first when critical_order is set.
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (P.level_bwd[x] != P.level_bwd[y]) return P.level_bwd[x] < P.level_bwd[y];
+        return opt.critical_order && P.level_fwd[x] > P.level_fwd[y];
+    });
     for (size_t lo = 0; lo < order.size();) {
         size_t hi = lo;
         while (hi < order.size() && P.level_bwd[order[hi]] == P.level_bwd[order[lo]]) ++hi;

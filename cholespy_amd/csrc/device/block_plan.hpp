@@ -80,6 +80,13 @@ struct BlockPlanOptions {
     // false: the wide list is a copy of the narrow one (batches, whose many
     // systems already give each level enough tasks).
     bool wide_list = true;
+    // Order of the blocks inside a level: true = the blocks with the longest
+    // chain of levels still ahead of them first (forward: most block
+    // ancestors; backward: most block levels below), so the tasks on the
+    // pass's critical path take the early tickets; false = block index.
+    // Blocks of one level are independent, so either order is a valid
+    // schedule.
+    bool critical_order = true;
     // Host threads for the dense block inverses (0 = hardware count, at most
     // 16). Batches already build one plan per thread and pass 1.
     int threads = 0;

@@ -168,6 +168,11 @@ struct cholespy_gpu_solver {
     int buffers_path = -1;                // path whose invariants the buffers hold
     size_t work_elems = 0;                // capacity of y and z in elements
     size_t part_elems = 0;
+    // Work buffers replaced by larger ones. They stay allocated until the
+    // handle is destroyed, so a graph captured before the growth still
+    // replays into memory that exists (every buffer holds the sentinel
+    // between solves, whatever layout it last had).
+    std::vector<void *> retired;
     DeviceBlockPlan plan;
     // Full row-compressed A in FP64 (residual of the refinement path).
     int *arowptr = nullptr, *acols = nullptr;
@@ -227,6 +232,7 @@ BlockPlanOptions plan_options(const BlockPlanOptions &base) {
     BlockPlanOptions opt = base;
     if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
+    if (const char *e = std::getenv("CHOLESPY_CRITICAL_ORDER")) opt.critical_order = std::atoi(e) != 0;
     // Task widths are limited to what the block kernels' lane layouts cover.
     auto width = [](const char *name, int &v, int hi) {
         if (const char *e = std::getenv(name)) v = std::min(hi, std::max(1, std::atoi(e)));
@@ -533,6 +539,8 @@ int cholespy_solver_destroy(cholespy_gpu_solver *s) {
     dev_free(s->part);
     dev_free(s->spart);
     dev_free(s->rbuf);
+    for (void *p : s->retired) (void)hipFree(p);
+    s->retired.clear();
     dev_free(s->arowptr);
     dev_free(s->acols);
     dev_free(s->avals);
@@ -771,31 +779,49 @@ namespace {
 
 // Makes the work buffers hold n x k elements in the layout and sentinel
 // state the current path expects.
+// Capacity for a buffer of `rows` rows that must now hold k columns, having
+// held `have` elements: at least twice the old capacity (at most 128
+// columns), so the retired buffers of a handle add up to less than its
+// final ones.
+size_t grown_capacity(size_t rows, int k, size_t have) {
+    const size_t need = rows * (size_t)k;
+    return std::max(need, std::min(2 * have, rows * (size_t)kMaxRhs));
+}
+
+template <typename T>
+void retire(cholespy_gpu_solver *s, T *&p) {
+    if (p) s->retired.push_back(static_cast<void *>(p));
+    p = nullptr;
+}
+
 int prepare_buffers(cholespy_gpu_solver *s, int k, hipStream_t st) {
     const int fbytes = s->fbytes;
     const size_t elems = (size_t)s->n * (size_t)k;
     if (elems > s->work_elems) {
-        dev_free(s->y);
-        dev_free(s->z);
-        dev_free(s->rbuf);
-        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), elems * (size_t)fbytes));
-        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), elems * (size_t)fbytes));
-        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->rbuf), elems * (size_t)fbytes));
-        s->work_elems = elems;
+        const size_t cap = grown_capacity((size_t)s->n, k, s->work_elems);
+        retire(s, s->y);
+        retire(s, s->z);
+        retire(s, s->rbuf);
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), cap * (size_t)fbytes));
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), cap * (size_t)fbytes));
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->rbuf), cap * (size_t)fbytes));
+        s->work_elems = cap;
         s->k_cur = 0;
     }
     const size_t pe = (size_t)s->plan.nslots * (size_t)k;
     if (s->path == CHOLESPY_PATH_BLOCKS && pe > s->part_elems) {
-        dev_free(s->part);
-        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->part), std::max<size_t>(pe, 1) * (size_t)fbytes));
-        s->part_elems = pe;
+        const size_t cap = grown_capacity((size_t)s->plan.nslots, k, s->part_elems);
+        retire(s, s->part);
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->part), std::max<size_t>(cap, 1) * (size_t)fbytes));
+        s->part_elems = cap;
         s->k_cur = 0;
     }
     const size_t se = (size_t)s->plan.nsplit_rows * (size_t)k;
     if (s->path == CHOLESPY_PATH_BLOCKS && se > s->spart_elems) {
-        dev_free(s->spart);
-        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->spart), std::max<size_t>(se, 1) * (size_t)fbytes));
-        s->spart_elems = se;
+        const size_t cap = grown_capacity((size_t)s->plan.nsplit_rows, k, s->spart_elems);
+        retire(s, s->spart);
+        HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->spart), std::max<size_t>(cap, 1) * (size_t)fbytes));
+        s->spart_elems = cap;
         s->k_cur = 0;
     }
     if (k != s->k_cur || s->buffers_path != s->path) {
@@ -1414,21 +1440,21 @@ int batch_solve(cholespy_gpu_solver *s, int fbytes, const int *n_rhs, const void
         HIP_TRY(hipMemcpy(s->sys_pbase, pb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s->sys_sbase, sb.data(), (size_t)ns * sizeof(long long), hipMemcpyHostToDevice));
         if (elems > s->work_elems) {
-            dev_free(s->y);
-            dev_free(s->z);
-            dev_free(s->rbuf);
+            retire(s, s->y);
+            retire(s, s->z);
+            retire(s, s->rbuf);
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->y), elems * (size_t)fbytes));
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->z), elems * (size_t)fbytes));
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->rbuf), elems * (size_t)fbytes));
             s->work_elems = elems;
         }
         if (pe > s->part_elems) {
-            dev_free(s->part);
+            retire(s, s->part);
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->part), pe * (size_t)fbytes));
             s->part_elems = pe;
         }
         if (se > s->spart_elems) {
-            dev_free(s->spart);
+            retire(s, s->spart);
             HIP_TRY(dev_alloc(reinterpret_cast<char **>(&s->spart), se * (size_t)fbytes));
             s->spart_elems = se;
         }

@@ -267,7 +267,7 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
              doc_constructor)
         .def(
             "solve",
-            [](Class &self, py::object b_o, py::object x_o) {
+            [](Class &self, py::object b_o, py::object x_o, uintptr_t stream) {
                 DLView b(b_o), x(x_o);
                 const bool is_f32 = std::is_same<Float, float>::value;
                 const auto good = [&](const DLView &v) {
@@ -309,10 +309,10 @@ void declare_solver(py::module_ &m, const char *name, const char *doc) {
                     py::gil_scoped_release release;
                     self.solve_cpu(n_rhs, bp, xp);
                 } else {
-                    self.solve_cuda(n_rhs, b.data(), x.data());
+                    self.solve_cuda(n_rhs, b.data(), x.data(), reinterpret_cast<void *>(stream));
                 }
             },
-            py::arg("b"), py::arg("x"), doc_solve)
+            py::arg("b"), py::arg("x"), py::arg("stream") = 0, doc_solve)
         .def(
             "refactorize",
             [](Class &self, py::object x_o) {

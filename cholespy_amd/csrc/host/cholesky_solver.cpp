@@ -68,14 +68,14 @@ CholeskySolver<Float>::~CholeskySolver() {
 }
 
 template <typename Float>
-void CholeskySolver<Float>::solve_cuda(int n_rhs, const void *b, void *x) {
+void CholeskySolver<Float>::solve_cuda(int n_rhs, const void *b, void *x, void *stream) {
     if (n_rhs > 128) throw std::invalid_argument("The number of RHS should be less than 128.");
     if (std::is_same<Float, float>::value)
         throw_on_error(cholespy_solver_solve_f32(gpu_, n_rhs, static_cast<const float *>(b),
-                                                 static_cast<float *>(x), nullptr));
+                                                 static_cast<float *>(x), stream));
     else
         throw_on_error(cholespy_solver_solve_f64(gpu_, n_rhs, static_cast<const double *>(b),
-                                                 static_cast<double *>(x), nullptr));
+                                                 static_cast<double *>(x), stream));
 }
 
 template <typename Float>

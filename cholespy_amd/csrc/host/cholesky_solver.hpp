@@ -29,8 +29,9 @@ public:
     CholeskySolver &operator=(const CholeskySolver &) = delete;
 
     // src/cholesky_solver.cpp:415-432. b and x are device addresses of
-    // row-major (n x n_rhs) arrays; asynchronous on the null stream.
-    void solve_cuda(int n_rhs, const void *b, void *x);
+    // row-major (n x n_rhs) arrays; asynchronous on `stream` (a
+    // hipStream_t, nullptr = the null stream, as the reference).
+    void solve_cuda(int n_rhs, const void *b, void *x, void *stream = nullptr);
 
     // src/cholesky_solver.cpp:434-463: row-major Float in/out, FP64 inside.
     void solve_cpu(int n_rhs, const Float *b, Float *x);

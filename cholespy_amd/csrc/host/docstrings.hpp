@@ -40,6 +40,12 @@ Parameters
         (n_rows, n_rhs), on the device the solver was built for. GPU solves
         accept at most 128 columns.
 - `x` - Output array with the shape and device of `b`; it is overwritten.
+- `stream` - GPU solves only: the HIP stream to launch on, as an integer
+        handle (`torch.cuda.current_stream().cuda_stream`). 0, the
+        default, is the null stream, as in cholespy. A solve on a stream
+        can be captured in a CUDA/HIP graph once the same number of
+        right-hand sides has been solved outside the capture (the first
+        solve at a new width sizes the work buffers).
 
 Both arrays must already have the solver's dtype (float32 for
 CholeskySolverF, float64 for CholeskySolverD) and be C-contiguous: `x` is

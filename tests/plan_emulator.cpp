@@ -82,6 +82,10 @@ bool parse_option(BlockPlanOptions &o, const std::string &kv) {
         o.wide_list = v != 0;
         return true;
     }
+    if (key == "critical_order") {
+        o.critical_order = v != 0;
+        return true;
+    }
     return false;
 }
 

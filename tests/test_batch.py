@@ -127,3 +127,34 @@ def test_batch_with_split_reductions(monkeypatch):
             off += n * k
             ref = oracle.solve(batch._factor(i), bs[i], np.float64)
             assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12, (i, n, k)
+
+
+@pytest.mark.gpu
+def test_full_size_batch_of_256_grid_systems():
+    """The batch workload at full size: 256 grid systems, n in [1k, 50k],
+    k_i in 1..128, FP32, one launch per pass. Every system's relative
+    residual in FP64 is small for an FP32 solve, and every 16th system is
+    within 2e-5 of the oracle on its own factor."""
+    systems = workloads.batch_systems(count=256, seed=0)
+    ks = [k for *_, k in systems]
+    print(f"batch of {len(systems)} systems, n total {sum(n for n, *_ in systems)}, sum k {sum(ks)}",
+          flush=True)
+    batch = BatchedSolverF([(n, idx[0], idx[1], v, MatrixType.COO) for n, v, idx, _ in systems])
+    rng = np.random.default_rng(9)
+    bs = [rng.uniform(-1, 1, (n, k)).astype(np.float32) for (n, *_), k in zip(systems, ks)]
+    b = torch.tensor(np.concatenate([x.ravel() for x in bs]), device="cuda")
+    x = torch.zeros_like(b)
+    batch.solve(ks, b, x)
+    torch.cuda.synchronize()
+    batch._check()
+    xs = x.cpu().numpy()
+    off = 0
+    for i, ((n, v, idx, _), k) in enumerate(zip(systems, ks)):
+        got = xs[off:off + n * k].reshape(n, k).astype(np.float64)
+        off += n * k
+        A = oracle.symmetric_from_lower(n, *oracle.normalized_lower_csc(n, idx[0], idx[1], v, "COO"))
+        res = np.linalg.norm(bs[i] - A @ got) / np.linalg.norm(bs[i])
+        assert res < 1e-5, (i, n, k, res)
+        if i % 16 == 0:
+            ref = oracle.solve(batch._factor(i), bs[i], np.float32)
+            assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-5, (i, n, k)

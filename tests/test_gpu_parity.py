@@ -232,3 +232,50 @@ def test_refined_to_tolerance(case):
     torch.cuda.synchronize()
     x2 = xt2.cpu().numpy()
     assert float((np.linalg.norm(b - A @ x2, axis=0) / np.linalg.norm(b, axis=0)).max()) > 1e-12
+
+
+@pytest.mark.parametrize("cls,dtype", [(CholeskySolverF, torch.float32), (CholeskySolverD, torch.float64)])
+def test_solve_on_a_stream_and_in_a_graph(cls, dtype):
+    """solve(b, x, stream=...) runs on the caller's stream, and a solve
+    captured in a graph replays bit-identically to eager solves of the same
+    right-hand sides, with eager solves (other widths too) between replays.
+    Both kernel families: k = 3 (narrow) and k = 32 (panel)."""
+    n, values, idx = CASES["ico5"]()
+    solver = _gpu_solver(cls, n, values, idx)
+    gen = torch.Generator(device=DEV).manual_seed(11)
+    for k in (3, 32):
+        b = torch.rand(n, k, device=DEV, dtype=dtype, generator=gen)
+        ref = torch.zeros_like(b)
+        solver.solve(b, ref)
+        torch.cuda.synchronize()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        x = torch.zeros_like(b)
+        with torch.cuda.stream(side):
+            solver.solve(b, x, stream=side.cuda_stream)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        solver._check()
+        assert torch.equal(x, ref), k
+        bg = b.clone()
+        xg = torch.zeros_like(b)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            solver.solve(bg, xg, stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        for rep in range(4):
+            bn = torch.rand(n, k, device=DEV, dtype=dtype, generator=gen)
+            want = torch.zeros_like(bn)
+            solver.solve(bn, want)
+            other = torch.rand(n, 5 + rep, device=DEV, dtype=dtype, generator=gen)
+            solver.solve(other, torch.zeros_like(other))
+            torch.cuda.synchronize()
+            bg.copy_(bn)
+            xg.zero_()
+            graph.replay()
+            torch.cuda.synchronize()
+            solver._check()
+            assert torch.equal(xg, want), (k, rep)
+        ref_np = oracle.solve(solver._factor(), bn.cpu().numpy(), np.float32 if dtype == torch.float32 else np.float64)
+        err = np.abs(xg.cpu().numpy() - ref_np).max() / np.abs(ref_np).max()
+        assert err < TOL[np.float32 if dtype == torch.float32 else np.float64], (k, err)

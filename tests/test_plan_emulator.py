@@ -73,6 +73,7 @@ OPTIONS = [
     ["split_entries=0"],
     ["wide_list=0"],
     ["leaf_max=128", "max_block=256"],
+    ["critical_order=0"],
 ]
 
 
