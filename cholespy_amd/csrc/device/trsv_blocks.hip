@@ -63,6 +63,10 @@ constexpr int kLeafWaves = kTaskThreads / kWave;
 #define CHOLESPY_KLEAFMU 16
 #endif
 constexpr int kLeafMU = CHOLESPY_KLEAFMU;   // M loads in flight per lane
+#ifndef CHOLESPY_KLEAFMU_B
+#define CHOLESPY_KLEAFMU_B CHOLESPY_KLEAFMU
+#endif
+constexpr int kLeafMUB = CHOLESPY_KLEAFMU_B;   // the same for backward leaf tasks
 
 // Orders this wavefront's LDS writes before its later LDS reads by other lanes.
 __device__ __forceinline__ void wave_sync() {
@@ -165,9 +169,9 @@ __device__ __forceinline__ void leaf_backward_task(const BlockPassArgs<F> &a, co
     // consecutive entries of the row-major packed M_B across the lanes).
     const int c = first + (lane & 31), h = lane >> 5;
     const bool active = c < last;
-    F mv[kLeafMU];
+    F mv[kLeafMUB];
 #pragma unroll
-    for (int j = 0; j < kLeafMU; ++j) {
+    for (int j = 0; j < kLeafMUB; ++j) {
         const int rr = first + h + 2 * j;
         mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
     }
@@ -193,9 +197,9 @@ __device__ __forceinline__ void leaf_backward_task(const BlockPassArgs<F> &a, co
     F acc[KT];
 #pragma unroll
     for (int t = 0; t < KT; ++t) acc[t] = F(0);
-    for (int j0 = 0;; j0 += kLeafMU) {
+    for (int j0 = 0;; j0 += kLeafMUB) {
 #pragma unroll
-        for (int j = 0; j < kLeafMU; ++j) {
+        for (int j = 0; j < kLeafMUB; ++j) {
             const int rr = first + h + 2 * (j0 + j);
             if (active && rr >= c && rr < rows) {
                 const F *vv = v + (rr - first) * KT;
@@ -203,10 +207,10 @@ __device__ __forceinline__ void leaf_backward_task(const BlockPassArgs<F> &a, co
                 for (int t = 0; t < KT; ++t) acc[t] = fma(mv[j], vv[t], acc[t]);
             }
         }
-        if (first + h + 2 * (j0 + kLeafMU) >= rows) break;
+        if (first + h + 2 * (j0 + kLeafMUB) >= rows) break;
 #pragma unroll
-        for (int j = 0; j < kLeafMU; ++j) {
-            const int rr = first + h + 2 * (j0 + kLeafMU + j);
+        for (int j = 0; j < kLeafMUB; ++j) {
+            const int rr = first + h + 2 * (j0 + kLeafMUB + j);
             mv[j] = active && rr >= c && rr < rows ? M[bwd_off(m, rr, c)] : F(0);
         }
     }
