@@ -466,6 +466,20 @@ def bench_precision(args):
                   torch.tensor(values, device=dev))
     sf = CholeskySolverF(n, ii, jj, xx, MatrixType.COO)
     log(f"FP32 solver (FP64 host factor + plan): {time.time() - t0:.1f}s, plan {sf._plan_info()}")
+    if args.profile:
+        # The counter passes: `steps` FP32 panel solves with the default
+        # schedule, then the 256 MiB copy that scales the counters.
+        b32 = torch.tensor(rhs, device=dev, dtype=torch.float32)
+        x32 = torch.zeros_like(b32)
+        ms = time_events(lambda: sf.solve(b32, x32), max(1, args.steps))
+        sf._check()
+        src = torch.ones(64 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": "f32 panel solve", "ms_per_step": ms}))
+        return
     t0 = time.time()
     tune = sf.autotune(k, force=True, write=False)
     log(f"autotune ({time.time() - t0:.1f}s): {tune['ms']:.4f} ms per FP32 solve (defaults {tune['default_ms']:.4f})")
