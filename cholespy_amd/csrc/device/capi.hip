@@ -1858,7 +1858,8 @@ extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *va
     };
     rc = eval(best, best_ms);
     const double default_ms = best_ms;
-    // One knob at a time; a candidate replaces the best when >= 1 % faster.
+    // One knob at a time; a candidate replaces the best when >= 0.5 % faster
+    // (repeated timings of one schedule agree to ~0.2 %).
     auto try_values = [&](const char *env, const std::vector<int> &values, auto set) {
         if (rc != CHOLESPY_OK || (env && std::getenv(env))) return;
         for (int v : values) {
@@ -1868,7 +1869,7 @@ extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *va
             double ms = 0;
             rc = eval(c, ms);
             if (rc != CHOLESPY_OK) return;
-            if (ms < 0.99 * best_ms) {
+            if (ms < 0.995 * best_ms) {
                 best = c;
                 best_ms = ms;
             }
@@ -1895,7 +1896,7 @@ extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *va
         for (int cap = 1024; cap <= 4096 && cap <= std::max(1024, chain); cap *= 2) caps.push_back(cap);
         try_values("CHOLESPY_MAX_BLOCK", caps, [](TuneConfig &c, int v) { c.opt.max_block = v; });
         // Leaf subtrees: larger ones can save a block level.
-        try_values("CHOLESPY_LEAF_MAX", {48, 96}, [](TuneConfig &c, int v) { c.opt.leaf_max = v; });
+        try_values("CHOLESPY_LEAF_MAX", {48, 56, 96}, [](TuneConfig &c, int v) { c.opt.leaf_max = v; });
         try_values("CHOLESPY_SPLIT_ENTRIES", {2048, 8192}, [](TuneConfig &c, int v) { c.opt.split_entries = v; });
         try_values("CHOLESPY_MAX_SPLIT", {4, 16}, [](TuneConfig &c, int v) { c.opt.max_split = v; });
         if (!wide) {
