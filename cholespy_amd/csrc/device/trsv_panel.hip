@@ -57,6 +57,16 @@ template <> struct Mfma<double> {
 // few accumulators and loads more steps ahead: kSteps(NT) MFMA steps have
 // their A (and B) loads issued together.
 template <int NT> constexpr int kStepsFor() { return NT <= 2 ? 8 : 4; }
+// Tasks whose B comes straight from global memory (large blocks forward,
+// every backward task) run a wavefront's tiles in batches of kBatchTiles,
+// each batch with kBatchSteps MFMA steps' A and B loads issued together:
+// the loads in flight no longer shrink as the tile count grows (a step
+// used to carry NT tiles' B registers whether or not the wavefront owned
+// them), so a task pays ~kBatchSteps times fewer dependent load rounds.
+constexpr int kBatchTiles = 2;
+// FP32: 8 steps (12 for the 8-tile kernels, whose occupancy floors leave
+// room); FP64: 4 (more cost the forward kernels a wave per SIMD).
+template <typename F, int NT> constexpr int batch_steps() { return sizeof(F) == 4 ? (NT >= 8 ? 12 : 8) : (NT <= 2 ? 8 : 4); }
 // Waves per SIMD the compiler must allow (amdgpu_waves_per_eu), chosen per
 // kernel as the most that compile with no register spills on gfx950. By
 // default the compiler keeps more accumulators in AGPRs and gets 2-3 waves;
@@ -260,6 +270,7 @@ void block_forward_panel(BlockPassArgs<F> a) {
     using Acc = typename Mfma<F>::Acc;
     constexpr int kMaxTiles = NT;
     constexpr int kSteps = kStepsFor<NT>();
+    constexpr int kBatchSteps = batch_steps<F, NT>();
     extern __shared__ __align__(16) unsigned char smem[];
     F *rv = reinterpret_cast<F *>(smem);
     __shared__ int s_next;
@@ -306,10 +317,25 @@ void block_forward_panel(BlockPassArgs<F> a) {
             const int glast = min(g0 + 16, last) - 1;
             const int cmax = glast < m ? glast + 1 : m;
             const int rowlim = row < m ? row + 1 : m;
-            Acc acc[kMaxTiles];
+            // Output of tile j (accumulator v of lane group kk: row g0 + row(kk, v)).
+            auto out_tile = [&](const Acc &acc_j, int j) {
+                const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
+                if (t >= k) return;
 #pragma unroll
-            for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
+                for (int v = 0; v < 4; ++v) {
+                    const int rho = g0 + Mfma<F>::row(kk, v);
+                    if (rho >= last) break;
+                    const F s = acc_j[v];
+                    if (kind == 3)
+                        publish(&sp[(long long)(srow + rho) * k + t], s);
+                    else
+                        finish(rho, t, s);
+                }
+            };
             if (kind == 0) {
+                Acc acc[kMaxTiles];
+#pragma unroll
+                for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
                 // Small block: r_B summed from the partials into LDS, in chunks.
                 for (int c0 = 0; c0 < m; c0 += chunk) {
                     const int c1 = min(m, c0 + chunk);
@@ -343,63 +369,68 @@ void block_forward_panel(BlockPassArgs<F> a) {
                     }
                     __syncthreads();
                 }
-            } else if (w.ntiles > 0) {
+                if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
+                if (w.ntiles > 0 && w.split == 0) {
+#pragma unroll
+                    for (int j = 0; j < kMaxTiles; ++j)
+                        if (j < w.ntiles) out_tile(acc[j], j);
+                }
+            } else {
                 // Large block: B = r_B read straight from rbuf (published by the
-                // block's gather tasks), no staging and no barriers; columns
-                // [lo, hi) only for a split task.
+                // block's gather tasks), no staging; columns [lo, hi) only for a
+                // split task. Tiles in batches (kBatchTiles, kBatchSteps).
                 using U = typename Bits<F>::U;
                 const int clo = kind == 3 ? task.lo : 0;
-                const int nsteps = (min(kind == 3 ? task.hi : m, cmax) - clo + 3) / 4;
+                const int chi = kind == 3 ? task.hi : m;
+                const int nsteps = (min(chi, cmax) - clo + 3) / 4;
                 const F *rb0 = rb + (long long)(a0 + clo) * k + (lane & 15) + 16 * w.tile0;
-                for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
-                    F av[kSteps];
-                    U bu[kSteps][kMaxTiles];
+                // Waves that split a reduction (one tile each) meet in
+                // reduce_splits once, idle ones included.
+                const int nbt = NT <= kBatchTiles ? 1 : max(w.ntiles, w.nsplit > 1 ? 1 : 0);
+                for (int tb = 0; tb < nbt; tb += kBatchTiles) {
+                    Acc acc[kBatchTiles];
 #pragma unroll
-                    for (int q = 0; q < kSteps; ++q) {
-                        const int s = s0 + q * w.nsplit;
-                        const int col = clo + 4 * s + kk;
-                        const int chi = kind == 3 ? task.hi : m;
-                        av[q] = (s < nsteps && row < last && col < rowlim && col < chi) ? M[fwd_off32(rows, col, row)]
-                                                                                       : F(0);
-                        const bool bok = s < nsteps && col < chi;
+                    for (int jj = 0; jj < kBatchTiles; ++jj) acc[jj] = Acc{};
+                    for (int s0 = w.split; w.ntiles > 0 && s0 < nsteps; s0 += w.nsplit * kBatchSteps) {
+                        F av[kBatchSteps];
+                        U bu[kBatchSteps][kBatchTiles];
 #pragma unroll
-                        for (int j = 0; j < kMaxTiles; ++j) {
-                            const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
-                            bu[q][j] = (j < w.ntiles && bok && t < k)
-                                           ? peek_bits(rb0 + (long long)(col - clo) * k + 16 * j * w.tstride) : U(0);
+                        for (int q = 0; q < kBatchSteps; ++q) {
+                            const int s = s0 + q * w.nsplit;
+                            const int col = clo + 4 * s + kk;
+                            av[q] = (s < nsteps && row < last && col < rowlim && col < chi) ? M[fwd_off32(rows, col, row)]
+                                                                                           : F(0);
+                            const bool bok = s < nsteps && col < chi;
+#pragma unroll
+                            for (int jj = 0; jj < kBatchTiles; ++jj) {
+                                const int j = tb + jj;
+                                const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
+                                bu[q][jj] = (j < w.ntiles && bok && t < k)
+                                                ? peek_bits(rb0 + (long long)(col - clo) * k + 16 * j * w.tstride) : U(0);
+                            }
+                        }
+#pragma unroll
+                        for (int q = 0; q < kBatchSteps; ++q) {
+                            const int s = s0 + q * w.nsplit;
+                            if (s < nsteps) {
+                                const int col = 4 * s + kk;   // relative to clo
+#pragma unroll
+                                for (int jj = 0; jj < kBatchTiles; ++jj) {
+                                    const int j = tb + jj;
+                                    if (j < w.ntiles)
+                                        acc[jj] = Mfma<F>::step(
+                                            av[q],
+                                            ready_value(rb0 + (long long)col * k + 16 * j * w.tstride, bu[q][jj], a.status, a.qa),
+                                            acc[jj]);
+                                }
+                            }
                         }
                     }
+                    if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
+                    if (w.ntiles > 0 && w.split == 0) {
 #pragma unroll
-                    for (int q = 0; q < kSteps; ++q) {
-                        const int s = s0 + q * w.nsplit;
-                        if (s < nsteps) {
-                            const int col = 4 * s + kk;   // relative to clo
-#pragma unroll
-                            for (int j = 0; j < kMaxTiles; ++j)
-                                if (j < w.ntiles)
-                                    acc[j] = Mfma<F>::step(
-                                        av[q], ready_value(rb0 + (long long)col * k + 16 * j * w.tstride, bu[q][j], a.status, a.qa),
-                                        acc[j]);
-                        }
-                    }
-                }
-            }
-            if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
-            if (w.ntiles > 0 && w.split == 0) {
-#pragma unroll
-                for (int j = 0; j < kMaxTiles; ++j) {
-                    if (j >= w.ntiles) break;
-                    const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
-                    if (t >= k) continue;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int rho = g0 + Mfma<F>::row(kk, v);
-                        if (rho >= last) break;
-                        const F s = acc[j][v];
-                        if (kind == 3)
-                            publish(&sp[(long long)(srow + rho) * k + t], s);
-                        else
-                            finish(rho, t, s);
+                        for (int jj = 0; jj < kBatchTiles; ++jj)
+                            if (tb + jj < w.ntiles) out_tile(acc[jj], tb + jj);
                     }
                 }
             }
@@ -425,6 +456,7 @@ void block_backward_panel(BlockPassArgs<F> a) {
     using Acc = typename Mfma<F>::Acc;
     constexpr int kMaxTiles = NT;
     constexpr int kSteps = kStepsFor<NT>();
+    constexpr int kBatchSteps = batch_steps<F, NT>();
     extern __shared__ __align__(16) unsigned char smem[];
     F *vs = reinterpret_cast<F *>(smem);
     __shared__ int s_next;
@@ -461,9 +493,6 @@ void block_backward_panel(BlockPassArgs<F> a) {
             const int cg0 = first + 16 * max(w.group, 0);
             const int c = cg0 + (lane & 15);   // A row (= column of M_B) of this lane
             const int kk = lane >> 4;          // A column / B row within a step
-            Acc acc[kMaxTiles];
-#pragma unroll
-            for (int j = 0; j < kMaxTiles; ++j) acc[j] = Acc{};
             // The R rows the task reads, [rb0, rb1), have their indices staged
             // in LDS (one load each, all in parallel) when they fit.
             const int he = kind == 1 ? task.hi : rows;
@@ -475,21 +504,27 @@ void block_backward_panel(BlockPassArgs<F> a) {
                 for (int r = rb0 + tid; r < he; r += kTaskThreads) srl[r - rb0] = a.rlist[r0 + r - m];
             __syncthreads();
             if (a.trace && tid == 0) a.trace[4 * (size_t)tk + 1] = wall_clock64();
-            if (w.ntiles > 0) {
-                // B = v[rho] = y_B[rho] (rho < m, complete since the forward
-                // pass) or z of R row rlist[rho - m] (published, waited on),
-                // read straight from global memory. Rows below the group's
-                // first column are zero in its columns.
-                using U = typename Bits<F>::U;
-                const int hs = kind == 1 ? max(cg0, task.lo) : cg0;   // rows [hs, he)
-                const int nsteps = (he - hs + 3) / 4;
-                const int toff = (lane & 15) + 16 * w.tile0;
-                for (int s0 = w.split; s0 < nsteps; s0 += w.nsplit * kSteps) {
-                    F av[kSteps];
-                    const F *bp[kSteps];
-                    bool pubr[kSteps];
+            // B = v[rho] = y_B[rho] (rho < m, complete since the forward pass)
+            // or z of R row rlist[rho - m] (published, waited on), read
+            // straight from global memory. Rows below the group's first
+            // column are zero in its columns. Tiles in batches (kBatchTiles,
+            // kBatchSteps); waves that split a reduction (one tile each) meet
+            // in reduce_splits once, idle ones included.
+            using U = typename Bits<F>::U;
+            const int hs = kind == 1 ? max(cg0, task.lo) : cg0;   // rows [hs, he)
+            const int nsteps = (he - hs + 3) / 4;
+            const int toff = (lane & 15) + 16 * w.tile0;
+            const int nbt = NT <= kBatchTiles ? 1 : max(w.ntiles, w.nsplit > 1 ? 1 : 0);
+            for (int tb = 0; tb < nbt; tb += kBatchTiles) {
+                Acc acc[kBatchTiles];
 #pragma unroll
-                    for (int q = 0; q < kSteps; ++q) {
+                for (int jj = 0; jj < kBatchTiles; ++jj) acc[jj] = Acc{};
+                for (int s0 = w.split; w.ntiles > 0 && s0 < nsteps; s0 += w.nsplit * kBatchSteps) {
+                    F av[kBatchSteps];
+                    const F *bp[kBatchSteps];
+                    bool pubr[kBatchSteps];
+#pragma unroll
+                    for (int q = 0; q < kBatchSteps; ++q) {
                         const int s = s0 + q * w.nsplit;
                         const int rho = hs + 4 * s + kk;
                         av[q] = (s < nsteps && c < last && rho < he && rho >= c) ? M[bwd_off32(m, rho, c)] : F(0);
@@ -500,51 +535,56 @@ void block_backward_panel(BlockPassArgs<F> a) {
                                              : zv + (long long)(staged ? srl[rho - rb0] : a.rlist[r0 + rho - m]) * k) +
                                     toff;
                     }
-                    U bu[kSteps][kMaxTiles];
+                    U bu[kBatchSteps][kBatchTiles];
 #pragma unroll
-                    for (int q = 0; q < kSteps; ++q)
+                    for (int q = 0; q < kBatchSteps; ++q)
 #pragma unroll
-                        for (int j = 0; j < kMaxTiles; ++j) {
+                        for (int jj = 0; jj < kBatchTiles; ++jj) {
+                            const int j = tb + jj;
                             const int t = toff + 16 * j * w.tstride;
                             const bool ok = j < w.ntiles && bp[q] && t < k;
-                            bu[q][j] = !ok ? U(0)
-                                           : (pubr[q] ? peek_bits(bp[q] + 16 * j * w.tstride)
-                                                      : as_bits(bp[q][16 * j * w.tstride]));
+                            bu[q][jj] = !ok ? U(0)
+                                            : (pubr[q] ? peek_bits(bp[q] + 16 * j * w.tstride)
+                                                       : as_bits(bp[q][16 * j * w.tstride]));
                         }
 #pragma unroll
-                    for (int q = 0; q < kSteps; ++q) {
+                    for (int q = 0; q < kBatchSteps; ++q) {
                         const int s = s0 + q * w.nsplit;
                         if (s < nsteps) {
 #pragma unroll
-                            for (int j = 0; j < kMaxTiles; ++j)
+                            for (int jj = 0; jj < kBatchTiles; ++jj) {
+                                const int j = tb + jj;
                                 if (j < w.ntiles) {
-                                    const F b = pubr[q] && bp[q] ? ready_value(bp[q] + 16 * j * w.tstride, bu[q][j], a.status, a.qa)
-                                                                 : from_bits<F>(bu[q][j]);
-                                    acc[j] = Mfma<F>::step(av[q], b, acc[j]);
+                                    const F b = pubr[q] && bp[q]
+                                                    ? ready_value(bp[q] + 16 * j * w.tstride, bu[q][jj], a.status, a.qa)
+                                                    : from_bits<F>(bu[q][jj]);
+                                    acc[jj] = Mfma<F>::step(av[q], b, acc[jj]);
                                 }
+                            }
                         }
                     }
                 }
-            }
-            if (w.nsplit > 1) {
-                __syncthreads();   // the staged R-row indices are no longer read
-                reduce_splits(vs, w, wave, lane, acc[0]);
-            }
-            if (w.ntiles > 0 && w.split == 0) {
+                if (w.nsplit > 1) {
+                    __syncthreads();   // the staged R-row indices are no longer read
+                    reduce_splits(vs, w, wave, lane, acc[0]);
+                }
+                if (w.ntiles > 0 && w.split == 0) {
 #pragma unroll
-                for (int j = 0; j < kMaxTiles; ++j) {
-                    if (j >= w.ntiles) break;
-                    const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
-                    if (t >= k) continue;
+                    for (int jj = 0; jj < kBatchTiles; ++jj) {
+                        const int j = tb + jj;
+                        if (j >= w.ntiles) break;
+                        const int t = 16 * (w.tile0 + j * w.tstride) + (lane & 15);
+                        if (t >= k) continue;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int cc = cg0 + Mfma<F>::row(kk, v);
-                        if (cc >= last) break;
-                        const F s = acc[j][v];
-                        if (kind == 1)
-                            publish(&sp[(long long)(srow + cc) * k + t], s);
-                        else
-                            finish(cc, t, s);
+                        for (int v = 0; v < 4; ++v) {
+                            const int cc = cg0 + Mfma<F>::row(kk, v);
+                            if (cc >= last) break;
+                            const F s = acc[jj][v];
+                            if (kind == 1)
+                                publish(&sp[(long long)(srow + cc) * k + t], s);
+                            else
+                                finish(cc, t, s);
+                        }
                     }
                 }
             }
