@@ -158,6 +158,79 @@ struct GatherStage {
     int slot[kStageRows][kStageSlots];
 };
 
+// The element phase of gather_wide for one staged round of nr rows at q0,
+// on 16-byte vectors: a thread takes kVecPer vectors (kVE consecutive
+// columns of one row) at a time and loads the b vector and kVecSlots
+// contribution vectors of each together: 4 (FP32) or 2 (FP64) elements per
+// load instruction and per address instead of one. A lane of a vector read
+// as the sentinel is waited on alone (ready_value), as in the scalar phase.
+// (2 vectors a thread spill registers in the forward panel kernels.)
+constexpr int kVecPer = 1;
+constexpr int kVecSlots = 4;
+template <typename F>
+__device__ __forceinline__ void gather_wide_vec(const BlockPassArgs<F> &a, const GatherStage &st, const F *bv,
+                                                const F *pv, F *rb, F *dst, int a0, int q0, int nr, int k, int stride,
+                                                bool to_rbuf, int r0) {
+    using U = typename Bits<F>::U;
+    constexpr int kVE = 16 / (int)sizeof(F);
+    typedef U V __attribute__((ext_vector_type(kVE)));
+    const int nv = k / kVE;
+    const int total = nr * nv;
+    for (int base = threadIdx.x; base < total; base += kTaskThreads * kVecPer) {
+        int ii[kVecPer], col[kVecPer], ncnt = 0;
+        F sum[kVecPer][kVE];
+#pragma unroll
+        for (int e = 0; e < kVecPer; ++e) {
+            const int idx = base + e * kTaskThreads;
+            const bool ok = idx < total;
+            ii[e] = ok ? idx / nv : -1;
+            col[e] = ok ? (idx - ii[e] * nv) * kVE : 0;
+            const V vb = ok ? *reinterpret_cast<const V *>(bv + (long long)st.perm[ii[e]] * k + col[e]) : V{};
+#pragma unroll
+            for (int c = 0; c < kVE; ++c) sum[e][c] = from_bits<F>(vb[c]);
+            if (ok) ncnt = max(ncnt, min(st.cnt[ii[e]], kStageSlots));
+        }
+        for (int g = 0; g < ncnt; g += kVecSlots) {
+            V us[kVecPer][kVecSlots];
+#pragma unroll
+            for (int e = 0; e < kVecPer; ++e)
+#pragma unroll
+                for (int j = 0; j < kVecSlots; ++j) {
+                    const int sl = ii[e] >= 0 && g + j < kStageSlots ? st.slot[ii[e]][g + j] : -1;
+                    us[e][j] = sl >= 0 ? *reinterpret_cast<const V *>(pv + (long long)sl * k + col[e]) : V{};
+                }
+#pragma unroll
+            for (int e = 0; e < kVecPer; ++e)
+#pragma unroll
+                for (int j = 0; j < kVecSlots; ++j) {
+                    const int sl = ii[e] >= 0 && g + j < kStageSlots ? st.slot[ii[e]][g + j] : -1;
+                    if (sl >= 0) {
+                        const F *p = pv + (long long)sl * k + col[e];
+#pragma unroll
+                        for (int c = 0; c < kVE; ++c) sum[e][c] += ready_value(p + c, us[e][j][c], a.status, a.qa);
+                    }
+                }
+        }
+#pragma unroll
+        for (int e = 0; e < kVecPer; ++e) {
+            const int i = ii[e];
+            if (i < 0) break;
+            for (int j = kStageSlots; j < st.cnt[i]; ++j) {   // rows with very many contributions
+                const F *p = &pv[(long long)a.cslot[st.first[i] + j] * k + col[e]];
+#pragma unroll
+                for (int c = 0; c < kVE; ++c) sum[e][c] += ready_value(p + c, peek_bits(p + c), a.status, a.qa);
+            }
+#pragma unroll
+            for (int c = 0; c < kVE; ++c) {
+                if (to_rbuf)
+                    publish(&rb[(long long)(a0 + q0 + i) * k + col[e] + c], sum[e][c]);
+                else
+                    dst[(q0 - r0 + i) * stride + col[e] + c] = sum[e][c];
+            }
+        }
+    }
+}
+
 template <typename F>
 __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherStage &st, const F *bv, const F *pv,
                                             F *rb, F *dst, int a0, int r0, int r1, int k, int stride,
@@ -165,6 +238,9 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
     using U = typename Bits<F>::U;
     constexpr int kStageElems = stage_elems<F>();
     const int tid = threadIdx.x;
+    // 16-byte vectors along the columns when k and the vector bases allow.
+    constexpr int kVE = 16 / (int)sizeof(F);
+    const bool vec_ok = k % kVE == 0 && ((reinterpret_cast<uintptr_t>(bv) | reinterpret_cast<uintptr_t>(pv)) & 15) == 0;
     for (int q0 = r0; q0 < r1; q0 += kStageRows) {
         const int nr = min(kStageRows, r1 - q0);
         if (tid < nr) {
@@ -187,6 +263,11 @@ __device__ __forceinline__ void gather_wide(const BlockPassArgs<F> &a, GatherSta
             }
         }
         __syncthreads();
+        if (vec_ok) {
+            gather_wide_vec(a, st, bv, pv, rb, dst, a0, q0, nr, k, stride, to_rbuf, r0);
+            __syncthreads();
+            continue;
+        }
         const int total = nr * k;
         for (int base = tid; base < total; base += kTaskThreads * kStageElems) {
             U ub[kStageElems];
