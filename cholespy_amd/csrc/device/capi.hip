@@ -233,7 +233,8 @@ BlockPlanOptions plan_options(const BlockPlanOptions &base) {
     if (const char *e = std::getenv("CHOLESPY_MAX_BLOCK")) opt.max_block = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_LEAF_MAX")) opt.leaf_max = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_CRITICAL_ORDER")) opt.critical_order = std::atoi(e) != 0;
-    // Task widths are limited to what the block kernels' lane layouts cover.
+    // Task widths are limited to what the block kernels' lane layouts cover
+    // (the panel kernels, which run the wide list: 64 rows or columns).
     auto width = [](const char *name, int &v, int hi) {
         if (const char *e = std::getenv(name)) v = std::min(hi, std::max(1, std::atoi(e)));
     };
@@ -250,7 +251,7 @@ BlockPlanOptions plan_options(const BlockPlanOptions &base) {
     width("CHOLESPY_WIDE_FWD_ROWS_BIG", opt.wide_fwd_rows_big, 64);
     if (const char *e = std::getenv("CHOLESPY_WIDE_SPLIT_GATHER")) opt.wide_split_gather = std::max(1, std::atoi(e));
     if (const char *e = std::getenv("CHOLESPY_WIDE_LEVEL_TASKS")) opt.wide_level_tasks = std::max(0, std::atoi(e));
-    width("CHOLESPY_WIDE_FINE_BWD_COLS_BIG", opt.wide_fine_bwd_cols_big, 32);
+    width("CHOLESPY_WIDE_FINE_BWD_COLS_BIG", opt.wide_fine_bwd_cols_big, 64);
     if (const char *e = std::getenv("CHOLESPY_WIDE_MIN_SPLIT_ENTRIES"))
         opt.wide_min_split_entries = std::max(1, std::atoi(e));
     return opt;

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 namespace cholespy_amd {
 
@@ -81,6 +82,13 @@ template <typename F, int NT, bool Forward> constexpr int kMinWavesFor() {
 }
 // Backward tasks stage the R-row indices they read in LDS (up to this many).
 constexpr int kStageRlist = 4096;
+// Entry i of an int array in LDS, read through an LDS-typed pointer: a load
+// through a generic pointer (what a choice between an LDS and a global array
+// compiles to) is a flat load, which counts against the vector-memory
+// counter too, so waiting for it would also wait for every M load in flight.
+__device__ __forceinline__ int lds_int(const int *p, int i) {
+    return ((const __attribute__((address_space(3))) int *)p)[i];
+}
 
 // Packed offsets of M_B entries (32-bit: one block's M_B has < 2^31 entries).
 __device__ __forceinline__ int fwd_off32(int rows, int c, int rho) {
@@ -600,6 +608,12 @@ void block_backward_panel(BlockPassArgs<F> a) {
                 Acc acc[kBatchTiles];
 #pragma unroll
                 for (int jj = 0; jj < kBatchTiles; ++jj) acc[jj] = Acc{};
+                // The batches, in two copies: R-row indices from LDS (staged) or
+                // from global memory. One copy choosing per row would join an
+                // LDS and a global load into one register, and the wait for it
+                // would drain every M load in flight.
+                auto batches = [&](auto staged_c) {
+                constexpr bool kStaged = decltype(staged_c)::value;
                 for (int s0 = w.split; w.ntiles > 0 && s0 < nsteps; s0 += w.nsplit * kBatchSteps) {
                     F av[kBatchSteps];
                     const F *bp[kBatchSteps];
@@ -613,7 +627,7 @@ void block_backward_panel(BlockPassArgs<F> a) {
                         bp[q] = nullptr;
                         if (s < nsteps && rho < he)
                             bp[q] = (rho < m ? yv + (long long)(a0 + rho) * k
-                                             : zv + (long long)(staged ? srl[rho - rb0] : a.rlist[r0 + rho - m]) * k) +
+                                             : zv + (long long)(kStaged ? lds_int(srl, rho - rb0) : a.rlist[r0 + rho - m]) * k) +
                                     toff;
                     }
                     U bu[kBatchSteps][kBatchTiles];
@@ -645,6 +659,11 @@ void block_backward_panel(BlockPassArgs<F> a) {
                         }
                     }
                 }
+                };
+                if (staged)
+                    batches(std::true_type{});
+                else
+                    batches(std::false_type{});
                 if (w.nsplit > 1) {
                     __syncthreads();   // the staged R-row indices are no longer read
                     reduce_splits(vs, w, wave, lane, acc[0]);
