@@ -1633,21 +1633,24 @@ std::string format_config(const TuneConfig &c) {
     char buf[320];
     std::snprintf(buf, sizeof buf,
                   "path=%d tile=%d fused=%d queues=%d max_block=%d split_entries=%d max_split=%d split_gather=%d "
-                  "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d leaf_max=%d",
+                  "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d leaf_max=%d "
+                  "wide_fine_bwd_cols_big=%d",
                   c.path, c.tile, c.fused, c.queues, c.opt.max_block, c.opt.split_entries, c.opt.max_split,
                   c.opt.split_gather, c.opt.wide_level_tasks, c.opt.wide_min_split_entries, c.opt.wide_fwd_rows_big,
-                  c.opt.leaf_max);
+                  c.opt.leaf_max, c.opt.wide_fine_bwd_cols_big);
     return buf;
 }
 
 bool parse_config(const std::string &text, TuneConfig &c) {
-    int v[12];
+    int v[13];
     if (std::sscanf(text.c_str(),
                     "path=%d tile=%d fused=%d queues=%d max_block=%d split_entries=%d max_split=%d split_gather=%d "
-                    "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d leaf_max=%d",
-                    &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11]) != 12)
+                    "wide_level_tasks=%d wide_min_split_entries=%d wide_fwd_rows_big=%d leaf_max=%d "
+                    "wide_fine_bwd_cols_big=%d",
+                    &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7], &v[8], &v[9], &v[10], &v[11],
+                    &v[12]) != 13)
         return false;
-    if (v[11] < 1 || v[11] > 4096) return false;
+    if (v[11] < 1 || v[11] > 4096 || v[12] < 1 || v[12] > 64) return false;
     if ((v[0] != CHOLESPY_PATH_ROWS && v[0] != CHOLESPY_PATH_BLOCKS) || v[2] < 0 || v[2] > 3 || v[3] < 1 ||
         v[3] > kMaxQueues || v[4] < 0 || v[5] < 0 || v[6] < 1 || v[6] > 16 || v[7] < 1 || v[8] < 0 ||
         v[9] < 1 || v[10] < 1 || v[10] > 64)
@@ -1664,6 +1667,7 @@ bool parse_config(const std::string &text, TuneConfig &c) {
     c.opt.wide_min_split_entries = v[9];
     c.opt.wide_fwd_rows_big = v[10];
     c.opt.leaf_max = v[11];
+    c.opt.wide_fine_bwd_cols_big = v[12];
     return true;
 }
 
@@ -1671,7 +1675,7 @@ bool same_plan(const BlockPlanOptions &a, const BlockPlanOptions &b) {
     return a.max_block == b.max_block && a.split_entries == b.split_entries && a.max_split == b.max_split &&
            a.split_gather == b.split_gather && a.wide_level_tasks == b.wide_level_tasks &&
            a.wide_min_split_entries == b.wide_min_split_entries && a.wide_fwd_rows_big == b.wide_fwd_rows_big &&
-           a.leaf_max == b.leaf_max;
+           a.leaf_max == b.leaf_max && a.wide_fine_bwd_cols_big == b.wide_fine_bwd_cols_big;
 }
 
 // Rebuilds and uploads the block plan for base options `base` (environment
@@ -1908,7 +1912,9 @@ extern "C" int cholespy_solver_autotune(cholespy_gpu_solver *s, const double *va
                        [](TuneConfig &c, int v) { c.opt.wide_level_tasks = v; });
             try_values("CHOLESPY_WIDE_MIN_SPLIT_ENTRIES", {256, 1024},
                        [](TuneConfig &c, int v) { c.opt.wide_min_split_entries = v; });
-            try_values("CHOLESPY_WIDE_FWD_ROWS_BIG", {32}, [](TuneConfig &c, int v) { c.opt.wide_fwd_rows_big = v; });
+            try_values("CHOLESPY_WIDE_FWD_ROWS_BIG", {32, 64}, [](TuneConfig &c, int v) { c.opt.wide_fwd_rows_big = v; });
+            try_values("CHOLESPY_WIDE_FINE_BWD_COLS_BIG", {32, 64},
+                       [](TuneConfig &c, int v) { c.opt.wide_fine_bwd_cols_big = v; });
         }
         try_values("CHOLESPY_BLOCK_QUEUES", {8}, [](TuneConfig &c, int v) { c.queues = v; });
     }

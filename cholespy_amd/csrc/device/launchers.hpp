@@ -89,8 +89,11 @@ inline int tile_slot(int tile) { return tile <= kNarrowTile ? 0 : (tile <= 32 ? 
 constexpr int kBlockLdsBytes = 32 * 1024;
 inline int block_chunk(int float_bytes, int tile) { return kBlockLdsBytes / (tile * float_bytes); }
 // Dynamic LDS of a panel launch (override: CHOLESPY_PANEL_LDS, 16-64 KiB);
-// at least 16 KiB, the backward kernel's staged R-row indices.
-constexpr int kPanelLdsBytes = 32 * 1024;
+// at least 16 KiB, the backward kernel's staged R-row indices. 28 KiB plus
+// the forward kernel's ~10 KiB of static gather staging lets four
+// workgroups share a CU's 160 KiB (32 KiB allowed three: the 64^3 k = 32
+// FP32 solve 1.72 -> 1.53 ms).
+constexpr int kPanelLdsBytes = 28 * 1024;
 // Task records as uploaded (kDevTaskInts ints, one 64-byte line): the
 // plan's kTaskInts ints (block, first, last, kind, lo, hi, sbase, split),
 // then the block's first row a0, size m, first partial slot r0, |R|, the

@@ -203,6 +203,36 @@ def test_split_reductions_match_oracle(case, cls, dtype, split_entries, monkeypa
         assert err < TOL[dtype], (case, k, err)
 
 
+@pytest.mark.parametrize("width", [32, 64])
+@pytest.mark.parametrize("case", ["ico5", "grid64"])
+@pytest.mark.parametrize("cls,dtype", [(CholeskySolverF, np.float32), (CholeskySolverD, np.float64)])
+def test_wide_panel_tasks_match_oracle(case, cls, dtype, width, monkeypatch):
+    """The MFMA panel kernels on 32- and 64-wide tasks of the large blocks
+    (forward rows, backward columns: two and four 16-wide groups, one per
+    wavefront at 64), split along their reductions, for 1 to 8 tiles of
+    right-hand sides."""
+    monkeypatch.setenv("CHOLESPY_WIDE_FINE_BWD_COLS_BIG", str(width))
+    monkeypatch.setenv("CHOLESPY_WIDE_FWD_ROWS_BIG", str(width))
+    monkeypatch.setenv("CHOLESPY_SPLIT_ENTRIES", "1024")
+    n, values, idx = CASES[case]()
+    solver = _gpu_solver(cls, n, values, idx)
+    fac = solver._factor()
+    tasks = solver._plan_arrays()
+    assert ((tasks["bwd_task_wide"][:, 2] - tasks["bwd_task_wide"][:, 1]) == width).any()
+    assert ((tasks["fwd_task_wide"][:, 2] - tasks["fwd_task_wide"][:, 1]) == width).any()
+    rng = np.random.default_rng(11)
+    for k in (5, 32, 33, 128):
+        b = rng.random((n, k)).astype(dtype)
+        bt = torch.tensor(b, device=DEV)
+        xt = torch.zeros_like(bt)
+        solver.solve(bt, xt)
+        torch.cuda.synchronize()
+        solver._check()
+        ref = oracle.solve(fac, b, dtype)
+        err = np.abs(xt.cpu().numpy() - ref).max() / np.abs(ref).max()
+        assert err < TOL[dtype], (case, k, err)
+
+
 @pytest.mark.parametrize("case", ["ico5", "grid64"])
 def test_refined_to_tolerance(case):
     """Refinement until the device-computed relative residual is <= tol: the

@@ -74,6 +74,7 @@ OPTIONS = [
     ["wide_list=0"],
     ["leaf_max=128", "max_block=256"],
     ["critical_order=0"],
+    ["wide_fine_bwd_cols_big=64", "wide_fwd_rows_big=64"],
 ]
 
 
