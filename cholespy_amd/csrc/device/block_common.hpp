@@ -331,7 +331,11 @@ __device__ __forceinline__ void gather_rows(const BlockPassArgs<F> &a, F *dst, i
             pub[j] = false;
             if (idx < total && t < kt) {
                 ad[j] = at(r0 + i, t, pub[j]);
-                u[j] = pub[j] ? peek_bits(ad[j]) : as_bits(*ad[j]);
+                // One load form for published and complete rows (complete
+                // rows never hold the sentinel): two forms picked per lane
+                // would be two loads into one register, and the compiler
+                // waits for the first before issuing the second.
+                u[j] = peek_bits(ad[j]);
             }
         }
 #pragma unroll

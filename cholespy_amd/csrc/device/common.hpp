@@ -52,18 +52,29 @@ __device__ __forceinline__ F from_bits(typename Bits<F>::U u) {
     return __builtin_bit_cast(F, u);
 }
 
+// Global-memory view of a pointer: accesses through it compile to global
+// (vector-memory) instructions. Where the compiler cannot prove a pointer's
+// address space, e.g. one picked from several arrays, it emits a flat access,
+// which also counts against the LDS counter; the wait for it then drains
+// every other access in flight. Every buffer the hand-off helpers below
+// touch is device (or mapped host) memory.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gmem(T *p) {
+    return (__attribute__((address_space(1))) T *)p;
+}
+
 // Agent-scope relaxed store of one value (its own publication).
 template <typename F>
 __device__ __forceinline__ void publish(F *p, F v) {
     using U = typename Bits<F>::U;
-    __hip_atomic_store(reinterpret_cast<U *>(p), as_bits(v), __ATOMIC_RELAXED,
+    __hip_atomic_store(gmem(reinterpret_cast<U *>(p)), as_bits(v), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <typename F>
 __device__ __forceinline__ typename Bits<F>::U load_bits(const F *p) {
     using U = typename Bits<F>::U;
-    return __hip_atomic_load(reinterpret_cast<const U *>(p), __ATOMIC_RELAXED,
+    return __hip_atomic_load(gmem(reinterpret_cast<const U *>(p)), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -77,15 +88,15 @@ __device__ __forceinline__ typename Bits<F>::U load_bits(const F *p) {
 template <typename F>
 __device__ __forceinline__ typename Bits<F>::U peek_bits(const F *p) {
     using U = typename Bits<F>::U;
-    return __hip_atomic_load(reinterpret_cast<const U *>(p), __ATOMIC_RELAXED,
+    return __hip_atomic_load(gmem(reinterpret_cast<const U *>(p)), __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 __device__ __forceinline__ int load_int(const int *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(gmem(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void store_int(int *p, int v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(gmem(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Dynamic work queues (below) and the watchdog's launch arguments.

@@ -638,9 +638,8 @@ void block_backward_panel(BlockPassArgs<F> a) {
                             const int j = tb + jj;
                             const int t = toff + 16 * j * w.tstride;
                             const bool ok = j < w.ntiles && bp[q] && t < k;
-                            bu[q][jj] = !ok ? U(0)
-                                            : (pubr[q] ? peek_bits(bp[q] + 16 * j * w.tstride)
-                                                       : as_bits(bp[q][16 * j * w.tstride]));
+                            // y rows never hold the sentinel: one load form for both.
+                            bu[q][jj] = !ok ? U(0) : peek_bits(bp[q] + 16 * j * w.tstride);
                         }
 #pragma unroll
                     for (int q = 0; q < kBatchSteps; ++q) {
