@@ -13,10 +13,11 @@
 //
 // Work split: a task's output is G groups of 16 rows (forward) or 16 columns
 // (backward), G <= 4, times ceil(k / 16) tiles of 16 right-hand sides. The 4
-// wavefronts of the workgroup take groups first, then tiles; when there are
-// fewer (group, tile) pairs than wavefronts, the wavefronts sharing a pair
-// split the 4-wide reduction steps between them and add their partial tiles
-// through LDS at the end.
+// wavefronts of the workgroup take groups first, then groups of up to two
+// consecutive tiles; when there are fewer (group, tile group) pairs than
+// wavefronts, the wavefronts sharing a pair split the 4-wide reduction steps
+// between them and add their partial tiles through LDS at the end, so a
+// wavefront's A loads are its own.
 //
 // The staged vector (r_B forward, [y_B ; z_R] backward) is held in LDS in
 // chunks of rows with a row stride of ceil(k / 16) * 16 (+16 when that is a
@@ -74,11 +75,13 @@ template <typename F, int NT> constexpr int batch_steps() { return sizeof(F) == 
 // more waves put more tasks in flight where many small tasks run at once
 // (the leaf levels, batches). FP32, 8 tiles: 224 -> 166 registers forward
 // (3 waves), 116 backward (4). FP32, up to 4 tiles: 4 waves forward, 5
-// backward. FP64 forward would spill and keeps the default; FP64 backward
-// with up to 4 tiles fits 4 waves.
+// backward. FP64 forward with 4 tiles: 3 waves (left alone the compiler
+// takes AGPRs and 2); other FP64 forward kernels would spill at more waves
+// and keep the default; FP64 backward with up to 4 tiles fits 4 waves.
 template <typename F, int NT, bool Forward> constexpr int kMinWavesFor() {
     if (sizeof(F) == 4) return NT >= 8 ? 3 : 4;
-    return !Forward && NT <= 4 ? 4 : 1;
+    if (!Forward) return NT <= 4 ? 4 : 1;
+    return NT == 4 ? 3 : 1;
 }
 // Backward tasks stage the R-row indices they read in LDS (up to this many).
 constexpr int kStageRlist = 4096;
@@ -109,24 +112,34 @@ __device__ __forceinline__ WaveWork wave_work(int wave, int width, int nT) {
     const int G = (width + 15) / 16;
     const int Gp = G <= 1 ? 1 : (G <= 2 ? 2 : 4);
     const int W = 4 / Gp;   // wavefronts per group
-    const int nTp = nT <= 1 ? 1 : (nT <= 2 ? 2 : (nT <= 4 ? 4 : 8));
+    // With fewer tile groups (up to kBatchTiles consecutive tiles) than
+    // wavefronts, each wavefront takes one group and the wavefronts left over
+    // split the group's reduction, so two wavefronts of a row group load the
+    // same A (M_B) entries only when there are several tile groups (one tile
+    // per wavefront with split pairs instead: the 64^3 k = 32 FP32 solve
+    // 1.49 ms against 1.18). Otherwise the tiles are dealt round.
+    const int tpw = nT < kBatchTiles ? nT : kBatchTiles;
+    const int TG = (nT + tpw - 1) / tpw;
     WaveWork w;
     w.group = wave / W;
     const int u = wave % W;
-    if (nTp >= W) {
+    if (TG >= W) {   // each wavefront takes tiles u, u + W, ... (in batches)
         w.tile0 = u;
         w.tstride = W;
         w.ntiles = u < nT ? (nT - u + W - 1) / W : 0;
         w.split = 0;
         w.nsplit = 1;
+        w.partner_stride = 1;
     } else {
-        w.tile0 = u % nTp;
-        w.tstride = nTp;
-        w.ntiles = w.tile0 < nT ? 1 : 0;
-        w.split = u / nTp;
-        w.nsplit = W / nTp;
+        const int TGp = TG <= 1 ? 1 : (TG <= 2 ? 2 : 4);
+        const int tg = u % TGp;
+        w.tile0 = tg * tpw;
+        w.tstride = 1;
+        w.ntiles = tg < TG ? min(tpw, nT - w.tile0) : 0;
+        w.split = u / TGp;
+        w.nsplit = W / TGp;
+        w.partner_stride = TGp;
     }
-    w.partner_stride = nTp;
     if (w.group >= G) {
         w.group = -1;
         w.ntiles = 0;
@@ -339,17 +352,26 @@ __device__ __forceinline__ void zero_tail(F *v, int len, int ks) {
     for (int idx = threadIdx.x; idx < 4 * ks; idx += kTaskThreads) v[len * ks + idx] = F(0);
 }
 
-// Adds the partial tiles of the other splits of this wavefront's pair (LDS
-// `red`, 64 x 4 values per wavefront). Every thread of the workgroup calls it.
-template <typename F, typename Acc>
-__device__ __forceinline__ void reduce_splits(F *red, const WaveWork &w, int wave, int lane, Acc &acc) {
+// Adds the partial tiles of the other splits of this wavefront's tile group
+// (LDS `red`, 64 x 4 values per wavefront and tile; a wavefront that splits
+// a reduction holds at most kBatchTiles tiles). Every thread of the
+// workgroup calls it.
+template <typename F, typename Acc, int N>
+__device__ __forceinline__ void reduce_splits(F *red, const WaveWork &w, int wave, int lane, Acc (&acc)[N]) {
+    constexpr int kTiles = N < kBatchTiles ? N : kBatchTiles;
     if (w.ntiles > 0 && w.split > 0)
-        for (int v = 0; v < 4; ++v) red[(wave * 64 + lane) * 4 + v] = acc[v];
+#pragma unroll
+        for (int j = 0; j < kTiles; ++j)
+            if (j < w.ntiles)
+                for (int v = 0; v < 4; ++v) red[((j * 4 + wave) * 64 + lane) * 4 + v] = acc[j][v];
     __syncthreads();
     if (w.ntiles > 0 && w.split == 0)
         for (int s = 1; s < w.nsplit; ++s) {
             const int pw = wave + s * w.partner_stride;
-            for (int v = 0; v < 4; ++v) acc[v] += red[(pw * 64 + lane) * 4 + v];
+#pragma unroll
+            for (int j = 0; j < kTiles; ++j)
+                if (j < w.ntiles)
+                    for (int v = 0; v < 4; ++v) acc[j][v] += red[((j * 4 + pw) * 64 + lane) * 4 + v];
         }
 }
 
@@ -458,7 +480,7 @@ void block_forward_panel(BlockPassArgs<F> a) {
                     }
                     __syncthreads();
                 }
-                if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
+                if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc);
                 if (w.ntiles > 0 && w.split == 0) {
 #pragma unroll
                     for (int j = 0; j < kMaxTiles; ++j)
@@ -473,8 +495,9 @@ void block_forward_panel(BlockPassArgs<F> a) {
                 const int chi = kind == 3 ? task.hi : m;
                 const int nsteps = (min(chi, cmax) - clo + 3) / 4;
                 const F *rb0 = rb + (long long)(a0 + clo) * k + (lane & 15) + 16 * w.tile0;
-                // Waves that split a reduction (one tile each) meet in
-                // reduce_splits once, idle ones included.
+                // Waves that split a reduction (at most kBatchTiles tiles
+                // each: one batch) meet in reduce_splits once, idle ones
+                // included.
                 const int nbt = NT <= kBatchTiles ? 1 : max(w.ntiles, w.nsplit > 1 ? 1 : 0);
                 for (int tb = 0; tb < nbt; tb += kBatchTiles) {
                     Acc acc[kBatchTiles];
@@ -515,7 +538,7 @@ void block_forward_panel(BlockPassArgs<F> a) {
                             }
                         }
                     }
-                    if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc[0]);
+                    if (w.nsplit > 1) reduce_splits(rv, w, wave, lane, acc);
                     if (w.ntiles > 0 && w.split == 0) {
 #pragma unroll
                         for (int jj = 0; jj < kBatchTiles; ++jj)
@@ -597,8 +620,9 @@ void block_backward_panel(BlockPassArgs<F> a) {
             // or z of R row rlist[rho - m] (published, waited on), read
             // straight from global memory. Rows below the group's first
             // column are zero in its columns. Tiles in batches (kBatchTiles,
-            // kBatchSteps); waves that split a reduction (one tile each) meet
-            // in reduce_splits once, idle ones included.
+            // kBatchSteps); waves that split a reduction (at most
+            // kBatchTiles tiles each: one batch) meet in reduce_splits once,
+            // idle ones included.
             using U = typename Bits<F>::U;
             const int hs = kind == 1 ? max(cg0, task.lo) : cg0;   // rows [hs, he)
             const int nsteps = (he - hs + 3) / 4;
@@ -665,7 +689,7 @@ void block_backward_panel(BlockPassArgs<F> a) {
                     batches(std::false_type{});
                 if (w.nsplit > 1) {
                     __syncthreads();   // the staged R-row indices are no longer read
-                    reduce_splits(vs, w, wave, lane, acc[0]);
+                    reduce_splits(vs, w, wave, lane, acc);
                 }
                 if (w.ntiles > 0 && w.split == 0) {
 #pragma unroll
