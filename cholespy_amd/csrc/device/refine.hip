@@ -36,7 +36,12 @@ __device__ __forceinline__ void add_column_squares(double *lsum, int k, int t, d
 
 // One thread per (row, right-hand side): consecutive threads share a row,
 // so cols / vals loads broadcast and x loads are contiguous across columns.
-// With norm2 set, also accumulates the residual's squared column norms.
+// A row's entries go in batches of kResBatch: their column indices and
+// values are loaded together, then their x entries together, so a batch
+// costs two dependent memory round trips instead of two per entry; the
+// products are still added in ascending column order. With norm2 set, also
+// accumulates the residual's squared column norms.
+constexpr int kResBatch = 8;
 __global__ void residual_f64(int n, int k, const int *__restrict__ rowptr, const int *__restrict__ cols,
                              const double *__restrict__ vals, const double *__restrict__ b,
                              const double *__restrict__ x, float *__restrict__ r32, double *norm2) {
@@ -51,7 +56,21 @@ __global__ void residual_f64(int n, int k, const int *__restrict__ rowptr, const
     for (size_t idx = idx0; idx < total && idx0 < stride; idx += stride) {
         const int row = (int)(idx / k), t = (int)(idx % k);
         double s = b[idx];
-        for (int p = rowptr[row]; p < rowptr[row + 1]; ++p) s = fma(-vals[p], x[(size_t)cols[p] * k + t], s);
+        const int p1 = rowptr[row + 1];
+        for (int p0 = rowptr[row]; p0 < p1; p0 += kResBatch) {
+            int c[kResBatch];
+            double v[kResBatch], xv[kResBatch];
+#pragma unroll
+            for (int j = 0; j < kResBatch; ++j) {
+                c[j] = p0 + j < p1 ? cols[p0 + j] : 0;
+                v[j] = p0 + j < p1 ? vals[p0 + j] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < kResBatch; ++j) xv[j] = p0 + j < p1 ? x[(size_t)c[j] * k + t] : 0.0;
+#pragma unroll
+            for (int j = 0; j < kResBatch; ++j)
+                if (p0 + j < p1) s = fma(-v[j], xv[j], s);
+        }
         r32[idx] = (float)s;
         mine = fma(s, s, mine);
     }
