@@ -118,7 +118,7 @@ __device__ __forceinline__ WaveWork wave_work(int wave, int width, int nT) {
     // same A (M_B) entries only when there are several tile groups (one tile
     // per wavefront with split pairs instead: the 64^3 k = 32 FP32 solve
     // 1.49 ms against 1.18). Otherwise the tiles are dealt round.
-    const int tpw = nT < kBatchTiles ? nT : kBatchTiles;
+    const int tpw = nT < kBatchTiles ? max(nT, 1) : kBatchTiles;
     const int TG = (nT + tpw - 1) / tpw;
     WaveWork w;
     w.group = wave / W;
