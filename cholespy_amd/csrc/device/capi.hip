@@ -40,6 +40,10 @@ namespace {
 // row passes, block passes) and two leaf-phase sets of kLeafQueueWords.
 // The device copy of the abort word follows them (its own 64-byte line).
 constexpr size_t kAllQueueWords = 6 * kQueueWords + 2 * kLeafQueueWords + 16;
+// Each set holds its queues' counters kQueueStride words apart and then the
+// done counter (finish_queues), so the set sizes follow the queue limits.
+static_assert(kQueueWords >= (size_t)(kMaxQueues + 1) * kQueueStride, "block queue counters overflow their set");
+static_assert(kLeafQueueWords >= (size_t)(kMaxLeafQueues + 1) * kQueueStride, "leaf queue counters overflow their set");
 
 thread_local std::string g_error;
 bool g_initialised = false;
