@@ -62,9 +62,10 @@ template <int NT> constexpr int kStepsFor() { return NT <= 2 ? 8 : 4; }
 // Tasks whose B comes straight from global memory (large blocks forward,
 // every backward task) run a wavefront's tiles in batches of kBatchTiles,
 // each batch with kBatchSteps MFMA steps' A and B loads issued together:
-// the loads in flight no longer shrink as the tile count grows (a step
-// used to carry NT tiles' B registers whether or not the wavefront owned
-// them), so a task pays ~kBatchSteps times fewer dependent load rounds.
+// a step holds B registers for the batch's tiles only, not for all NT, so
+// the loads in flight do not shrink as the tile count grows, and a task
+// pays ~kBatchSteps times fewer dependent load rounds than one step at a
+// time.
 constexpr int kBatchTiles = 2;
 // FP32: 8 steps (12 for the 8-tile kernels, whose occupancy floors leave
 // room); FP64: 4 (more cost the forward kernels a wave per SIMD).
