@@ -41,6 +41,11 @@ min(k, nproc) threads (each column is an independent solve).
 --profile runs only the headline path for `steps` solves and then a plain
 256 MiB device copy (the known-size kernel that scales the counters); it is
 the program the rocprofv3 counter passes run.
+
+After the headline (outside its timed region) the default run starts the
+other workloads (--workload precision, batch, grad, reference) as child
+processes and records their lines in brief under config.other_workloads;
+--no-extra skips them.
 """
 from __future__ import annotations
 
@@ -120,6 +125,8 @@ def main():
     ap.add_argument("--workload", choices=["smoothing", "batch", "precision", "grad", "reference"],
                     default="smoothing")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the other workloads (precision, batch, grad, reference) after the headline")
     ap.add_argument("--profile", action="store_true",
                     help="headline path only, then a 256 MiB copy (for rocprofv3 counter passes)")
     args = ap.parse_args()
@@ -425,6 +432,7 @@ def main():
             "block_plan": plan,
             "autotune": {p: {kk: v for kk, v in r.items() if kk != "key"} for p, r in tuned.items()},
             "reference_benchmark": ref_bench,
+            "other_workloads": other_workloads(args),
             "parallelism": "single GPU",
         },
         "roofline": {
@@ -442,6 +450,45 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
+
+
+def other_workloads(args):
+    """The other configurations, each in a child process after the headline
+    (outside its timed region): their bench lines in brief, so one default
+    run records every configuration's numbers."""
+    if args.no_extra or args.profile:
+        return None
+    import subprocess
+
+    runs = {"precision": ["--steps", "100"], "batch": ["--steps", "50"], "grad": ["--steps", "50"],
+            "reference": ["--steps", "200"]}
+    keep = {"precision": ("refine_steps", "ms", "refined_vs_f64", "adaptive"),
+            "batch": ("n_total", "sum_k", "ms_batched", "ms_graph", "ms_per_system", "batched_vs_per_system",
+                      "rel_err_f32_vs_oracle"),
+            "grad": ("ms_f32", "ms_f64", "entry_grad_rel_diff_f32_vs_f64"),
+            "reference": ("ms_defaults", "rel_err_vs_fp64_host", "reference_ms_titan_rtx")}
+    out = {}
+    for name, extra in runs.items():
+        t0 = time.time()
+        try:
+            r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--workload", name, *extra],
+                               capture_output=True, text=True, timeout=600)
+            line = [l for l in r.stdout.strip().splitlines() if l.startswith("{")][-1] if r.returncode == 0 else None
+        except (subprocess.TimeoutExpired, IndexError) as e:
+            r, line = None, None
+            log(f"{name} workload: {e}")
+        if line is None:
+            out[name] = {"error": (r.stderr[-400:] if r is not None else "no result")}
+            continue
+        d = json.loads(line)
+        cfg = d.get("config", {})
+        out[name] = {"workload": cfg.get("workload"), "value": d.get("value"), "unit": d.get("unit"),
+                     "ms_per_step": d.get("ms_per_step"), "dtype": d.get("dtype"),
+                     "vs_baseline": d.get("vs_baseline"), "roofline": d.get("roofline"),
+                     **{kk: cfg[kk] for kk in keep[name] if kk in cfg},
+                     "wall_s": round(time.time() - t0, 1)}
+        log(f"{name} workload: {d.get('ms_per_step'):.4f} ms per step ({time.time() - t0:.0f} s)")
+    return out
 
 
 def bench_precision(args):
